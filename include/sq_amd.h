@@ -1,0 +1,151 @@
+/*
+ * sq_amd.h -- C ABI of the MI355X spectral-quadrature (SQ) engine, libsqamd.so.
+ *
+ * The engine replaces SQMiniApp's per-grid-node Lanczos hot path
+ * (NodesAndWeights_LSSGQ and what it calls).  Plain pointers, sizes and error
+ * codes only; no PETSc, torch or HIP types cross this boundary.  Unless a
+ * function says "device", every pointer argument is host memory.
+ *
+ * Conventions taken from the reference:
+ *  - grid node (K,J,I) is a global FD-grid index, i fastest;
+ *  - a node's cube is the n_x*n_y*n_z box with origin (I-ceil(Rcut/h_x), ...),
+ *    n_d = 2*ceil(Rcut/h_d)+1 (lsdft_readfiles.cc:442-444,
+ *    lsdft_matvec.cc:38-45), i fastest;
+ *  - per-node outputs are rows in LIpos order (k-zcor)*lx*ly + (j-ycor)*lx +
+ *    (i-xcor) over the rank box (lsdft_scf.cc:67).
+ * Errors: every int-returning function returns SQ_OK (0) or an sq_status code
+ * and sets a thread-local message readable with sq_last_error(); nothing calls
+ * exit() (the reference exits on these conditions, e.g. lsdft_tools.cc:20-26,
+ * 950-953).
+ */
+#ifndef SQ_AMD_H
+#define SQ_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct sq_ctx sq_ctx;
+
+enum sq_status {
+    SQ_OK = 0,
+    SQ_ERR_IO = 1,        /* a deck file could not be opened */
+    SQ_ERR_FORMAT = 2,    /* a deck file is malformed */
+    SQ_ERR_RANGE = 3,     /* spline abscissa outside the radial table, or a value out of range */
+    SQ_ERR_ARG = 4,       /* bad argument */
+    SQ_ERR_QL = 5,        /* more than 50 QL sweeps for one eigenvalue (lsdft_tools.cc:950-953) */
+    SQ_ERR_ALLOC = 6,     /* host or device allocation failed */
+    SQ_ERR_BREAKDOWN = 7, /* Lanczos b_k == 0: the reference divides by it (lsdft_scf.cc:131,147) */
+    SQ_ERR_DEVICE = 8,    /* a HIP call failed */
+    SQ_ERR_STATE = 9      /* call out of order (e.g. results requested before a run) */
+};
+
+/* Nonlocal operator applied inside H. */
+enum sq_nonlocal_mode {
+    SQ_NL_REF_COLLAPSED = 0, /* Mult_NonlocalVector (lsdft_matvec.cc:25-149): one alpha per atom */
+    SQ_NL_KB = 1             /* Mult_HamiltonianVector (lsdft_matvec.cc:155-293): alpha per (atom,l,m) */
+};
+
+/* Arithmetic of the Lanczos recurrence. */
+enum sq_precision {
+    SQ_FP64 = 0, /* plain form: FP64 vectors, FP64 dot products */
+    SQ_FP32 = 1  /* FP32 vectors and stencil, compensated dot products; FP64 QL and quadrature */
+};
+
+typedef struct {
+    int phantom_atom;        /* 1: emulate the reference's extra atom read past the position
+                                array (endPosSC = 3*nAtoms, lsdft_initObjs.cc:3038) at phantom_pos */
+    double phantom_pos[3];
+    int rank_box[6];         /* owned node box (xcor,ycor,zcor,lx,ly,lz); lx<=0 means the whole grid */
+    int nqp_override;        /* >0 replaces LANCZOS_DEGREE */
+    int device;              /* HIP device ordinal; < 0 builds the host tables only (no device
+                                calls; the run entry points then return SQ_ERR_STATE) */
+} sq_options;
+
+typedef struct {
+    int npts[3], nloc[3], halfwidth[3], order, nqp, ntype, nAtoms, noetot;
+    int n_images, n_sc_atoms, ip_length, n_nodes, nc;
+    int rank_box[6], ext_origin[3], ext_dims[3];
+    double range[3], delta[3], dV, beta, rcut;
+    double coeffs[3][11];    /* -(1/2) Laplacian FD weights, lsdft_readfiles.cc:453-476 */
+    double gamma[8][4];      /* Gamma_l per type, lsdft_initObjs.cc:822-869 */
+    double cutoff[8][3];     /* pseudocharge cutoff, lsdft_initObjs.cc:628-817 */
+    long long chi_entries;   /* total projector entries over all images */
+} sq_info;
+
+/* Message of the last failing call on this thread ("" if none). */
+const char *sq_last_error(void);
+
+/* Default options: no phantom atom, whole grid, deck Lanczos degree, device 0. */
+void sq_default_options(sq_options *opt);
+
+/* Host setup + device upload.  Replaces Initialize (lsdft_initObjs.cc:36-93),
+ * GetInfluencingAtoms_Nonlocal (lsdft_initObjs.cc:1671-1946),
+ * CalculateNonlocalProjectors (lsdft_nl.cc:141-208) and NonlocalProjectors
+ * (lsdft_nl.cc:6-130).  Reads <dir>/<name>.inpt, <dir>/<name>.ion and
+ * <dir>/pseudopotential/<PSEUDOPOTENTIAL_FILE>.  V_eff starts as the
+ * reference's constant 0.333 (lsdft_scf.cc:71). */
+int sq_create_from_deck(const char *dir, const char *name, const sq_options *opt, sq_ctx **out);
+
+int sq_destroy(sq_ctx *ctx);
+
+int sq_get_info(sq_ctx *ctx, sq_info *info);
+
+/* V_eff on the global grid (npts[0]*npts[1]*npts[2], i fastest), wrapped
+ * periodically in grid index on each cube; field == NULL sets the constant
+ * veff_const everywhere.  Replaces VecSet(VeffNodal) + MatDiagonalSet
+ * (lsdft_scf.cc:71-75). */
+int sq_set_veff(sq_ctx *ctx, const double *field, double veff_const);
+
+/* The batched hot path: NodesAndWeights_LSSGQ (lsdft_scf.cc:32-82) for the
+ * rank-box nodes [node_begin, node_begin+node_count): N_qp+1 H applications
+ * per node (CalculateSpectralNodesAndWeights, lsdft_scf.cc:87-164) and the QL
+ * nodes/weights (TridiagEigenVecSolve_NodesAndWeights, lsdft_tools.cc:912-1026).
+ * Results stay on the device (see sq_get_nodes_and_weights). */
+int sq_run(sq_ctx *ctx, int nonlocal_mode, int precision, long node_begin, long node_count);
+
+/* Copy the last sq_run results for its nodes: lambda/w [node_count][nqp],
+ * lmin/lmax [node_count] (any pointer may be NULL). */
+int sq_get_nodes_and_weights(sq_ctx *ctx, double *lambda, double *w, double *lmin, double *lmax);
+
+/* Lanczos coefficients a[node][0..nqp], b[node][0..nqp] of the last sq_run. */
+int sq_get_lanczos(sq_ctx *ctx, double *a, double *b);
+
+/* sq_run + sq_get_nodes_and_weights. */
+int sq_nodes_and_weights(sq_ctx *ctx, int nonlocal_mode, int precision, long node_begin, long node_count,
+                         double *lambda, double *w, double *lmin, double *lmax);
+
+/* Device time of the last sq_run's Lanczos kernel and QL kernel, in ms,
+ * measured with HIP events on the context's stream. */
+int sq_last_timing(sq_ctx *ctx, double *lanczos_ms, double *ql_ms);
+
+/* One H application on node (Kp,Jp,Ip), on the device:
+ * y = (L + diag V_eff) x + V_nl x.  Replaces MatMult(LapPlusVeffOprloc) +
+ * Mult_NonlocalVector / Mult_HamiltonianVector (lsdft_scf.cc:121-123,
+ * lsdft_matvec.cc:25-293).  x, y: host vectors of nc entries. */
+int sq_apply_h(sq_ctx *ctx, int nonlocal_mode, int Kp, int Jp, int Ip, const double *x, double *y);
+
+/* Batched QL nodes and weights on the device for host tridiagonals:
+ * diag[count][n], subdiag[count][n] (subdiag[.][n-1] ignored), outputs
+ * lambda/w [count][n], lmin/lmax [count].  TridiagEigenVecSolve_NodesAndWeights,
+ * lsdft_tools.cc:912-1026. */
+int sq_tridiag_nodes_weights(sq_ctx *ctx, const double *diag, const double *subdiag, int n, long count,
+                             double *lambda, double *w, double *lmin, double *lmax);
+
+/* Density, Fermi level and band/entropy energy from the last sq_run, which must
+ * have covered every node of the rank box.  The reference declares these only
+ * (CalculateDensity_lssgq, constraint_lssgq, FermiDirac, findRootBrent,
+ * SystemEnergy_Calc_lssgq: lsdft.h:98-102,209).  rho[n_nodes] may be NULL.
+ * nelec <= 0 uses the deck's electron count; beta <= 0 uses the deck's BETA. */
+int sq_density_fermi_energy(sq_ctx *ctx, double nelec, double beta, double *lambda_f, double *rho,
+                            double *e_band, double *minus_ts);
+
+/* Projector image c of the setup: center (3), box (xs,ys,zs,xe,ye,ze),
+ * supercell atom index, type, entries (Nlm*box points), and the flattened
+ * chi (k,j,i outer, (l,m) inner, as lsdft_nl.cc:81-123) if chi != NULL. */
+int sq_get_image(sq_ctx *ctx, int c, double *center, int *box, int *atom, int *type, int *entries, double *chi);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

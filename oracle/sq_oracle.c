@@ -1,0 +1,1274 @@
+/*
+ * sq_oracle.c -- CPU FP64 restatement of SQMiniApp's spectral-quadrature hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the checker: only tests/, the smoke()
+ * entry of __graft_entry__.py and the cpu_baseline leg of bench.py load it.  The
+ * product (sqminiapp_amd/, libsqamd.so) never links, loads or calls it.
+ *
+ * Every function follows the reference file:line cited above it, in the same
+ * order of floating-point operations where that order is visible in the
+ * reference source.  Where the reference delegates arithmetic to PETSc/BLAS
+ * (VecDot, VecNorm, the SeqAIJ MatMult row sum) the oracle uses the plain
+ * sequential sum; the PETSc version is unpinned, so that rounding is
+ * "parity unpinned" at the ulp level (see the notes next to each use).
+ *
+ * Build: oracle/Makefile (gcc -O2 -ffp-contract=off -fopenmp).
+ */
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <omp.h>
+
+#define SQO_MAXTYPE 8
+#define SQO_MAXORDER 10
+
+enum { SQO_OK = 0, SQO_ERR_IO = 1, SQO_ERR_FORMAT = 2, SQO_ERR_RANGE = 3,
+       SQO_ERR_ARG = 4, SQO_ERR_QL = 5, SQO_ERR_ALLOC = 6, SQO_ERR_BREAKDOWN = 7 };
+
+typedef struct {
+    int size, lmax;
+    double *r, *V[4], *U[4], *uu, *Vloc;
+    double rc[4];
+    double *UDeltaV[4], *YD_UDeltaV[4], *YD_U[4], *YD_Vloc, *YD_uu;
+    double Gamma[4];
+} sqo_psd;
+
+typedef struct {
+    int n;
+    double *X0, *Y0, *Z0;
+    int *xs, *ys, *zs, *xe, *ye, *ze, *xindex;
+    int *ndim;
+    double **chi;
+    int **lId, **mId, **kId, **jId, **iId;
+} sqo_overlap;
+
+typedef struct sqo_sys {
+    /* .inpt (lsdft_readfiles.cc:62-102 defaults, 148-331 keywords) */
+    int order, bc, nqp, ntype;
+    double range[3], beta, rcut, tol_psc;
+    int npts[3];
+    int localPsd[SQO_MAXTYPE];
+    char psdName[SQO_MAXTYPE][64];
+    /* derived (lsdft_readfiles.cc:356-497) */
+    double delta[3], dV;
+    int nloc[3];
+    double coeffs[3][SQO_MAXORDER + 1], stencil_coeffs[3][SQO_MAXORDER + 1];
+    /* .ion (lsdft_readfiles.cc:677-789) */
+    int nAtoms, noetot;
+    char atomType[SQO_MAXTYPE][16];
+    int noe[SQO_MAXTYPE], noa[SQO_MAXTYPE], startPos[SQO_MAXTYPE], endPos[SQO_MAXTYPE];
+    double *atompos;
+    sqo_psd psd[SQO_MAXTYPE];
+    double cutoff[SQO_MAXTYPE][3];
+    /* supercell list (lsdft_initObjs.cc:3031-3041) */
+    double *atomposSC;
+    int startPosSC[SQO_MAXTYPE], endPosSC[SQO_MAXTYPE];
+    int IP_length, nIP, *IP_index;
+    /* nonlocal overlap for the current rank box */
+    int box[6], have_nl;
+    sqo_overlap ovl[SQO_MAXTYPE];
+    /* cube Laplacian in CSR (lsdft_initObjs.cc:393-475) */
+    int nc, *rowptr, *colidx;
+    double *val;
+    int center;
+} sqo_sys;
+
+static double *dalloc(size_t n) { return (double *)calloc(n ? n : 1, sizeof(double)); }
+static int *ialloc(size_t n) { return (int *)calloc(n ? n : 1, sizeof(int)); }
+
+/* ------------------------------------------------------------------ tools */
+
+/* fract(n,k) = (n!)^2/((n-k)!(n+k)!)   -- lsdft_tools.cc:519-531 */
+double sqo_fract(int n, int k)
+{
+    double Nr = 1, Dr = 1;
+    for (int i = n - k + 1; i <= n; i++) Nr *= i;
+    for (int i = n + 1; i <= n + k; i++) Dr *= i;
+    return Nr / Dr;
+}
+
+/* tridiag_gen: Gauss elimination, lsdft_tools.cc:122-151 */
+static int tridiag_gen(double *A, double *B, double *C, double *D, int len)
+{
+    double b, *F = dalloc(len);
+    b = B[0];
+    D[0] = D[0] / b;
+    for (int i = 1; i < len; i++) {
+        F[i] = C[i - 1] / b;
+        b = B[i] - A[i] * F[i];
+        if (b == 0) { free(F); return SQO_ERR_RANGE; }
+        D[i] = (D[i] - D[i - 1] * A[i]) / b;
+    }
+    for (int i = len - 2; i >= 0; i--) D[i] -= (D[i + 1] * F[i + 1]);
+    free(F);
+    return SQO_OK;
+}
+
+/* getYD_gen: not-a-knot cubic-spline slopes (Wolberg), lsdft_tools.cc:75-115 */
+int sqo_spline_yd(const double *X, const double *Y, double *YD, int len)
+{
+    int i;
+    double h0, h1, r0, r1;
+    double *A = dalloc(len), *B = dalloc(len), *C = dalloc(len);
+    h0 = X[1] - X[0];
+    h1 = X[2] - X[1];
+    r0 = (Y[1] - Y[0]) / h0;
+    r1 = (Y[2] - Y[1]) / h1;
+    B[0] = h1 * (h0 + h1);
+    C[0] = (h0 + h1) * (h0 + h1);
+    YD[0] = r0 * (3 * h0 * h1 + 2 * h1 * h1) + r1 * h0 * h0;
+    for (i = 1; i < len - 1; i++) {
+        h0 = X[i] - X[i - 1];
+        h1 = X[i + 1] - X[i];
+        r0 = (Y[i] - Y[i - 1]) / h0;
+        r1 = (Y[i + 1] - Y[i]) / h1;
+        A[i] = h1;
+        B[i] = 2 * (h0 + h1);
+        C[i] = h0;
+        YD[i] = 3 * (r0 * h1 + r1 * h0);
+    }
+    A[i] = (h0 + h1) * (h0 + h1);
+    B[i] = h0 * (h0 + h1);
+    YD[i] = r0 * h1 * h1 + r1 * (3 * h0 * h1 + 2 * h0 * h0);
+    int rc = tridiag_gen(A, B, C, YD, len);
+    free(A); free(B); free(C);
+    return rc;
+}
+
+/* SplineEvaluate for one abscissa, lsdft_tools.cc:15-68.  The reference finds
+ * the interval with a linear scan from j=0 ("for(;j<len1 && p2>X1[j];j++);
+ * if(p2<X1[j]) j--;"); the first X1[j] >= p2 is found here by bisection, which
+ * selects the same j.  Out-of-range abscissae are an error (the reference
+ * calls exit(1), lsdft_tools.cc:20-26). */
+int sqo_spline_eval(const double *X1, const double *Y1, int len1, double p2, const double *YD, double *out)
+{
+    if (p2 < X1[0] || p2 > X1[len1 - 1]) return SQO_ERR_RANGE;
+    int lo = 0, hi = len1; /* first index with X1[idx] >= p2 */
+    while (lo < hi) {
+        int mid = (lo + hi) >> 1;
+        if (p2 > X1[mid]) lo = mid + 1; else hi = mid;
+    }
+    int j = lo;
+    if (p2 < X1[j]) j--;
+    double p1 = X1[j];
+    double dx = 1.0 / (X1[j + 1] - X1[j]);
+    double dy = (Y1[j + 1] - Y1[j]) * dx;
+    double A0 = Y1[j];
+    double A1 = YD[j];
+    double A2 = dx * (3.0 * dy - 2.0 * YD[j] - YD[j + 1]);
+    double A3 = dx * dx * (-2.0 * dy + YD[j] + YD[j + 1]);
+    double x = p2 - p1;
+    *out = ((A3 * x + A2) * x + A1) * x + A0;
+    return SQO_OK;
+}
+
+/* RealSphericalHarmonic, lsdft_tools.cc:156-367, for l<=3 (the .pot reader
+ * stores l=0..3 only, lsdft_readfiles.cc:949-1096, so l=4..6 are unreachable on
+ * the path).  The coefficients are the reference's 15-digit truncations. */
+double sqo_ylm(double x, double y, double z, int l, int m, int *err)
+{
+    const double C00 = 0.282094791773878;
+    const double C1m1 = 0.488602511902920, C10 = 0.488602511902920, C1p1 = 0.488602511902920;
+    const double C2m2 = 1.092548430592079, C2m1 = 1.092548430592079, C20 = 0.315391565252520;
+    const double C2p1 = 1.092548430592079, C2p2 = 0.546274215296040;
+    const double C3m3 = 0.590043589926644, C3m2 = 2.890611442640554, C3m1 = 0.457045799464466;
+    const double C30 = 0.373176332590115, C3p1 = 0.457045799464466, C3p2 = 1.445305721320277;
+    const double C3p3 = 0.590043589926644;
+    double r = sqrt(x * x + y * y + z * z);
+    double SH = 0.0;
+    if (err) *err = 0;
+    if (l == 0) SH = C00;
+    if (r >= 1e-10) {
+        if (l == 0) SH = C00;
+        else if (l == 1) {
+            if (m == -1) SH = C1m1 * (y / r);
+            else if (m == 0) SH = C10 * (z / r);
+            else if (m == 1) SH = C1p1 * (x / r);
+            else if (err) *err = SQO_ERR_ARG;
+        } else if (l == 2) {
+            if (m == -2) SH = C2m2 * (x * y) / (r * r);
+            else if (m == -1) SH = C2m1 * (y * z) / (r * r);
+            else if (m == 0) SH = C20 * (-x * x - y * y + 2.0 * z * z) / (r * r);
+            else if (m == 1) SH = C2p1 * (z * x) / (r * r);
+            else if (m == 2) SH = C2p2 * (x * x - y * y) / (r * r);
+            else if (err) *err = SQO_ERR_ARG;
+        } else if (l == 3) {
+            if (m == -3) SH = C3m3 * (3 * x * x - y * y) * y / (r * r * r);
+            else if (m == -2) SH = C3m2 * (x * y * z) / (r * r * r);
+            else if (m == -1) SH = C3m1 * y * (4 * z * z - x * x - y * y) / (r * r * r);
+            else if (m == 0) SH = C30 * z * (2 * z * z - 3 * x * x - 3 * y * y) / (r * r * r);
+            else if (m == 1) SH = C3p1 * x * (4 * z * z - x * x - y * y) / (r * r * r);
+            else if (m == 2) SH = C3p2 * z * (x * x - y * y) / (r * r * r);
+            else if (m == 3) SH = C3p3 * x * (x * x - 3 * y * y) / (r * r * r);
+            else if (err) *err = SQO_ERR_ARG;
+        } else if (err) *err = SQO_ERR_ARG;
+    }
+    return SH;
+}
+
+/* TridiagEigenVecSolve_NodesAndWeights, lsdft_tools.cc:912-1026: implicit QL
+ * (Numerical Recipes tqli) with the full eigenvector matrix z (z = I at the
+ * start); lambda_i = d_i (unsorted), w_i = z[0][i]^2. */
+int sqo_tridiag_nodes_weights(const double *diag, const double *subdiag, int n,
+                              double *lambda, double *w, double *lmin, double *lmax)
+{
+    int m, l, iter, i, k;
+    double s, r, p, g, f, dd, c, b;
+    double *d = dalloc(n), *e = dalloc(n), *z = dalloc((size_t)n * n);
+#define Z(a, bb) z[(size_t)(a) * n + (bb)]
+    for (i = 0; i < n; i++) {
+        d[i] = diag[i];
+        e[i] = subdiag[i];
+        Z(i, i) = 1.0;
+    }
+    e[n - 1] = 0.0;
+    for (l = 0; l <= n - 1; l++) {
+        iter = 0;
+        do {
+            for (m = l; m <= n - 2; m++) {
+                dd = fabs(d[m]) + fabs(d[m + 1]);
+                if ((double)(fabs(e[m]) + dd) == dd) break;
+            }
+            if (m != l) {
+                if (iter++ == 50) { free(d); free(e); free(z); return SQO_ERR_QL; }
+                g = (d[l + 1] - d[l]) / (2.0 * e[l]);
+                r = sqrt(g * g + 1.0);
+                g = d[m] - d[l] + e[l] / (g + (g >= 0.0 ? fabs(r) : -fabs(r)));
+                s = c = 1.0;
+                p = 0.0;
+                for (i = m - 1; i >= l; i--) {
+                    f = s * e[i];
+                    b = c * e[i];
+                    e[i + 1] = (r = sqrt(g * g + f * f));
+                    if (r == 0.0) {
+                        d[i + 1] -= p;
+                        e[m] = 0.0;
+                        break;
+                    }
+                    s = f / r;
+                    c = g / r;
+                    g = d[i + 1] - p;
+                    r = (d[i] - g) * s + 2.0 * c * b;
+                    d[i + 1] = g + (p = s * r);
+                    g = c * r - b;
+                    for (k = 0; k < n; k++) {
+                        f = Z(k, i + 1);
+                        Z(k, i + 1) = s * Z(k, i) + c * f;
+                        Z(k, i) = c * Z(k, i) - s * f;
+                    }
+                }
+                if (r == 0.0 && i >= l) continue;
+                d[l] -= p;
+                e[l] = g;
+                e[m] = 0.0;
+            }
+        } while (m != l);
+    }
+    double emin = d[0], emax = d[0];
+    for (i = 0; i < n; i++) {
+        lambda[i] = d[i];
+        w[i] = Z(0, i) * Z(0, i);
+        if (d[i] > emax) emax = d[i];
+        else if (d[i] < emin) emin = d[i];
+    }
+    if (lmin) *lmin = emin;
+    if (lmax) *lmax = emax;
+#undef Z
+    free(d); free(e); free(z);
+    return SQO_OK;
+}
+
+/* ------------------------------------------------------------ input files */
+
+/* Read_Parameters, lsdft_readfiles.cc:21-672 (the keywords the path uses). */
+static int read_inpt(sqo_sys *S, const char *path)
+{
+    FILE *f = fopen(path, "rb");
+    if (!f) return SQO_ERR_IO;
+    S->ntype = -1; S->order = 6; S->beta = 1000; S->bc = 1; S->nqp = 50; S->rcut = 12.0;
+    S->tol_psc = 1e-6;
+    char str[256];
+    do {
+        if (fscanf(f, "%255s", str) != 1) break;
+        if (!strcmp(str, "FD_ORDER:")) { if (fscanf(f, "%d", &S->order) != 1) goto bad; S->order /= 2; }
+        else if (!strcmp(str, "BOUNDARY_CONDITION:")) { if (fscanf(f, "%d", &S->bc) != 1) goto bad; }
+        else if (!strcmp(str, "CELL:")) { if (fscanf(f, "%lf %lf %lf", &S->range[0], &S->range[1], &S->range[2]) != 3) goto bad; }
+        else if (!strcmp(str, "FD_GRID:")) { if (fscanf(f, "%d %d %d", &S->npts[0], &S->npts[1], &S->npts[2]) != 3) goto bad; }
+        else if (!strcmp(str, "BETA:")) { if (fscanf(f, "%lf", &S->beta) != 1) goto bad; }
+        else if (!strcmp(str, "RADIUS_TRUNC:")) { if (fscanf(f, "%lf", &S->rcut) != 1) goto bad; }
+        else if (!strcmp(str, "LANCZOS_DEGREE:")) { if (fscanf(f, "%d", &S->nqp) != 1) goto bad; }
+        else if (!strcmp(str, "TOL_PSEUDOCHARGE:")) { if (fscanf(f, "%lf", &S->tol_psc) != 1) goto bad; }
+        else if (!strcmp(str, "NTYPE:")) {
+            if (fscanf(f, "%d", &S->ntype) != 1 || S->ntype < 1 || S->ntype > SQO_MAXTYPE) goto bad;
+        } else if (!strcmp(str, "PSEUDOPOTENTIAL_LOCAL:")) {
+            if (S->ntype <= 0) goto bad;
+            for (int i = 0; i < S->ntype; i++) if (fscanf(f, "%d", &S->localPsd[i]) != 1) goto bad;
+        } else if (!strcmp(str, "PSEUDOPOTENTIAL_FILE:")) {
+            if (S->ntype <= 0) goto bad;
+            for (int i = 0; i < S->ntype; i++) if (fscanf(f, "%63s", S->psdName[i]) != 1) goto bad;
+        }
+    } while (!feof(f));
+    fclose(f);
+    if (S->ntype <= 0 || S->order < 1 || S->order > SQO_MAXORDER) return SQO_ERR_FORMAT;
+    if (S->bc != 1) return SQO_ERR_ARG; /* BC=1 is the only case the shipped deck and the path use */
+    /* grid spacing, lsdft_readfiles.cc:356-360 */
+    for (int d = 0; d < 3; d++) S->delta[d] = 2.0 * S->range[d] / (S->npts[d] - 1);
+    /* cube edge, lsdft_readfiles.cc:442-444 */
+    for (int d = 0; d < 3; d++) S->nloc[d] = 2 * (int)ceil(S->rcut / S->delta[d]) + 1;
+    S->dV = S->delta[0] * S->delta[1] * S->delta[2];
+    /* -(1/2) Laplacian FD weights, lsdft_readfiles.cc:453-476 */
+    double c0 = 0;
+    for (int a = 1; a <= S->order; a++) c0 += ((double)1.0 / (a * a));
+    for (int d = 0; d < 3; d++) {
+        S->coeffs[d][0] = c0;
+        S->coeffs[d][0] *= ((double)1.0 / (S->delta[d] * S->delta[d]));
+        for (int p = 1; p <= S->order; p++)
+            S->coeffs[d][p] = (double)(-1 * pow(-1, p + 1) * sqo_fract(S->order, p) /
+                                       (p * p * S->delta[d] * S->delta[d]));
+        for (int p = 0; p <= S->order; p++) S->stencil_coeffs[d][p] = S->coeffs[d][p] / (2.0 * M_PI);
+    }
+    return SQO_OK;
+bad:
+    fclose(f);
+    return SQO_ERR_FORMAT;
+}
+
+/* Read_Ion, lsdft_readfiles.cc:677-789 */
+static int read_ion(sqo_sys *S, const char *path)
+{
+    FILE *f = fopen(path, "rb");
+    if (!f) return SQO_ERR_IO;
+    if (fscanf(f, "%d", &S->nAtoms) != 1 || S->nAtoms < 0) { fclose(f); return SQO_ERR_FORMAT; }
+    S->atompos = dalloc(3 * (size_t)S->nAtoms + 3);
+    int at = 0, index = 0;
+    S->noetot = 0;
+    for (;;) {
+        if (at >= SQO_MAXTYPE) break;
+        if (fscanf(f, "%15s", S->atomType[at]) != 1) break;
+        if (fscanf(f, "%d", &S->noe[at]) != 1) break;
+        if (fscanf(f, "%d", &S->noa[at]) != 1) break;
+        S->startPos[at] = index;
+        for (int k = 0; k < S->noa[at]; k++) {
+            S->noetot += S->noe[at];
+            if (index + 3 > 3 * S->nAtoms) { fclose(f); return SQO_ERR_FORMAT; }
+            if (fscanf(f, "%lf %lf %lf", &S->atompos[index], &S->atompos[index + 1], &S->atompos[index + 2]) != 3) {
+                fclose(f); return SQO_ERR_FORMAT;
+            }
+            index += 3;
+        }
+        S->endPos[at] = index - 1;
+        at++;
+    }
+    fclose(f);
+    if (at != S->ntype) return SQO_ERR_FORMAT; /* lsdft_readfiles.cc:741-745 */
+    return SQO_OK;
+}
+
+/* Read_Pseudopotential (Troullier-Martins format), lsdft_readfiles.cc:858-1101.
+ * Units: V_l = 0.5*(r V_l in Ry)/r (Ry->Ha and r*V -> V), U_l = u_l/r,
+ * uu = rho/(4 pi r^2); r=0 is prepended with V(0)=V(r_1), U(0)=U(r_1). */
+static int read_pot(sqo_psd *P, const char *path, const char *atomType, int lloc)
+{
+    char str[60];
+    double value, rc;
+    int l, count = 0;
+    FILE *f = fopen(path, "rb");
+    if (!f) return SQO_ERR_IO;
+    if (fscanf(f, "%59s", str) != 1 || strcmp(str, atomType)) { fclose(f); return SQO_ERR_FORMAT; }
+    do { if (!fgets(str, 60, f)) { fclose(f); return SQO_ERR_FORMAT; } } while (strcmp(str, " Radial grid follows\n"));
+    do {
+        if (fscanf(f, "%59s", str) != 1) { fclose(f); return SQO_ERR_FORMAT; }
+        count++;
+    } while (strcmp(str, "Pseudopotential") != 0);
+    fclose(f);
+    count = count - 1;
+    P->size = count + 1;
+    P->r = dalloc(P->size);
+    P->uu = dalloc(P->size);
+    P->Vloc = dalloc(P->size);
+    for (int i = 0; i < 4; i++) { P->V[i] = dalloc(P->size); P->U[i] = dalloc(P->size); P->rc[i] = 0.0; }
+    /* rc[3] is never written for a 3-channel potential in the reference (UB,
+     * lsdft_readfiles.cc:914); it is taken as 0 here. */
+    f = fopen(path, "rb");
+    if (!f) return SQO_ERR_IO;
+    do { if (!fgets(str, 60, f)) { fclose(f); return SQO_ERR_FORMAT; } } while (strcmp(str, " Radial grid follows\n"));
+    P->r[0] = 0.0;
+    for (int i = 1; i <= count; i++) { if (fscanf(f, "%lf", &value) != 1) { fclose(f); return SQO_ERR_FORMAT; } P->r[i] = value; }
+    int lineCtr = 0;
+    while (strcmp(str, " Pseudopotential follows (l on next line)\n")) {
+        if (!fgets(str, 60, f)) { fclose(f); return SQO_ERR_FORMAT; }
+        lineCtr++;
+    }
+    P->lmax = -1;
+    while (strcmp(str, " Pseudopotential follows (l on next line)\n") == 0) {
+        if (fscanf(f, "%d", &l) != 1 || l < 0 || l > 3) { fclose(f); return SQO_ERR_FORMAT; }
+        P->lmax = l;
+        for (int i = 1; i <= count; i++) {
+            if (fscanf(f, "%lf", &value) != 1) { fclose(f); return SQO_ERR_FORMAT; }
+            value = 0.5 * value / (P->r[i]);
+            P->V[l][i] = value;
+            if (lloc == l) P->Vloc[i] = value;
+        }
+        P->V[l][0] = P->V[l][1];
+        if (lloc == l) P->Vloc[0] = P->V[l][1];
+        for (int i = 0; i < lineCtr; i++) if (!fgets(str, 60, f)) { fclose(f); return SQO_ERR_FORMAT; }
+    }
+    while (strcmp(str, " Valence charge follows\n")) if (!fgets(str, 60, f)) { fclose(f); return SQO_ERR_FORMAT; }
+    while (strcmp(str, " Valence charge follows\n") == 0) {
+        for (int i = 1; i <= count; i++) {
+            if (fscanf(f, "%lf", &value) != 1) { fclose(f); return SQO_ERR_FORMAT; }
+            value = value / (4 * M_PI * P->r[i] * P->r[i]);
+            P->uu[i] = value;
+        }
+        P->uu[0] = 0.0;
+        for (int i = 0; i < lineCtr; i++) if (!fgets(str, 60, f)) { fclose(f); return SQO_ERR_FORMAT; }
+    }
+    while (strcmp(str, " Pseudo-wave-function follows (l, zelect, rc)\n") == 0) {
+        if (fscanf(f, "%d %lf %lf", &l, &value, &rc) != 3 || l < 0 || l > 3) { fclose(f); return SQO_ERR_FORMAT; }
+        P->rc[l] = rc;
+        for (int i = 1; i <= count; i++) {
+            if (fscanf(f, "%lf", &value) != 1) { fclose(f); return SQO_ERR_FORMAT; }
+            value = value / P->r[i];
+            P->U[l][i] = value;
+        }
+        P->U[l][0] = P->U[l][1];
+        for (int i = 0; i < lineCtr; i++) {
+            if (feof(f)) break;
+            if (!fgets(str, 60, f)) break;
+        }
+    }
+    fclose(f);
+    if (P->lmax < 0 || lloc > P->lmax) return SQO_ERR_FORMAT;
+    return SQO_OK;
+}
+
+/* Calculate_SplineFit, lsdft_initObjs.cc:576-622 */
+static int spline_fit(sqo_psd *P)
+{
+    int rc = 0;
+    P->YD_Vloc = dalloc(P->size);
+    P->YD_uu = dalloc(P->size);
+    rc |= sqo_spline_yd(P->r, P->Vloc, P->YD_Vloc, P->size);
+    rc |= sqo_spline_yd(P->r, P->uu, P->YD_uu, P->size);
+    for (int l = 0; l <= P->lmax; l++) {
+        P->UDeltaV[l] = dalloc(P->size);
+        P->YD_UDeltaV[l] = dalloc(P->size);
+        P->YD_U[l] = dalloc(P->size);
+        for (int i = 0; i < P->size; i++) P->UDeltaV[l][i] = P->U[l][i] * (P->V[l][i] - P->Vloc[i]);
+        rc |= sqo_spline_yd(P->r, P->UDeltaV[l], P->YD_UDeltaV[l], P->size);
+        rc |= sqo_spline_yd(P->r, P->U[l], P->YD_U[l], P->size);
+    }
+    return rc ? SQO_ERR_RANGE : SQO_OK;
+}
+
+/* Calculate_NonlocalCoefficients, lsdft_initObjs.cc:822-869: Gamma_l = 1/D_l
+ * with D_l a rectangle rule (dr = 1e-6) of U_l*dV_l*U_l*r^2 up to rc_l + max h,
+ * minus half the last sample, divided by dV. */
+static int nonlocal_gamma(sqo_sys *S, sqo_psd *P, int lloc)
+{
+    const double dr = 1e-6;
+    double a = S->delta[0], b = S->delta[1], c = S->delta[2];
+    double delta = (a > b && a > c ? a : b > c ? b : c);
+    for (int l = 0; l <= P->lmax; l++) {
+        P->Gamma[l] = 0.0;
+        double rtemp = dr, Dexact = 0.0, UlDelVl = 0.0, Ul = 0.0;
+        if (l != lloc) {
+            double Rcut = P->rc[l];
+            while (rtemp <= (Rcut + delta)) {
+                if (sqo_spline_eval(P->r, P->UDeltaV[l], P->size, rtemp, P->YD_UDeltaV[l], &UlDelVl)) return SQO_ERR_RANGE;
+                if (sqo_spline_eval(P->r, P->U[l], P->size, rtemp, P->YD_U[l], &Ul)) return SQO_ERR_RANGE;
+                Dexact += UlDelVl * Ul * rtemp * rtemp * dr;
+                rtemp = rtemp + dr;
+            }
+            Dexact = (Dexact - 0.5 * UlDelVl * Ul * (rtemp - dr) * (rtemp - dr) * dr) / (S->dV);
+            P->Gamma[l] = 1.0 / Dexact;
+        }
+    }
+    return SQO_OK;
+}
+
+/* Calculate_PseudochargeCutoff, lsdft_initObjs.cc:628-817: the pseudocharge
+ * b = -(1/4pi) Lap_h V_loc on an octant grid around an atom at the origin;
+ * the radius grows by 1 bohr from rc[lloc] until |int b + Z|/Z <= tol or 15. */
+static int pseudocharge_cutoff(sqo_sys *S, int at)
+{
+    const double MaxRadius = 15.0;
+    sqo_psd *P = &S->psd[at];
+    int o = S->order;
+    int Np[3];
+    for (int d = 0; d < 3; d++) Np[d] = (int)ceil(MaxRadius / S->delta[d]);
+    int Ex = Np[0] + 2 * o, Ey = Np[1] + 2 * o, Ez = Np[2] + 2 * o;
+    double *Vps = dalloc((size_t)Ex * Ey * Ez);
+    double *BJ = dalloc((size_t)Np[0] * Np[1] * Np[2]);
+#define VPS(k, j, i) Vps[((size_t)(k) * Ey + (j)) * Ex + (i)]
+#define BJA(k, j, i) BJ[((size_t)(k) * Np[1] + (j)) * Np[0] + (i)]
+    for (int k = 0; k < Ez; k++)
+        for (int j = 0; j < Ey; j++)
+            for (int i = 0; i < Ex; i++) {
+                double x = (i - o) * S->delta[0], y = (j - o) * S->delta[1], z = (k - o) * S->delta[2];
+                double r = sqrt(x * x + y * y + z * z);
+                if (sqo_spline_eval(P->r, P->Vloc, P->size, r, P->YD_Vloc, &VPS(k, j, i))) {
+                    free(Vps); free(BJ); return SQO_ERR_RANGE;
+                }
+            }
+    const double *scx = S->stencil_coeffs[0], *scy = S->stencil_coeffs[1], *scz = S->stencil_coeffs[2];
+    for (int k = o; k < Np[2] + o; k++)
+        for (int j = o; j < Np[1] + o; j++)
+            for (int i = o; i < Np[0] + o; i++) {
+                double v = VPS(k, j, i) * (scx[0] + scy[0] + scz[0]);
+                for (int a = 1; a <= o; a++)
+                    v += (VPS(k, j, i - a) + VPS(k, j, i + a)) * scx[a] +
+                         (VPS(k, j - a, i) + VPS(k, j + a, i)) * scy[a] +
+                         (VPS(k - a, j, i) + VPS(k + a, j, i)) * scz[a];
+                BJA(k - o, j - o, i - o) = v;
+            }
+    double Rcut = P->rc[S->localPsd[at]], error;
+    int Ncx, Ncy, Ncz;
+    do {
+        Ncx = (int)ceil(Rcut / S->delta[0]);
+        Ncy = (int)ceil(Rcut / S->delta[1]);
+        Ncz = (int)ceil(Rcut / S->delta[2]);
+        double Bint = 0.0;
+        for (int k = 0; k < Ncz; k++)
+            for (int j = 0; j < Ncy; j++)
+                for (int i = 0; i < Ncx; i++) {
+                    double val = BJA(k, j, i);
+                    if (k == 0) val = 0.5 * val;
+                    if (k == Ncz - 1) val = 0.5 * val;
+                    if (j == 0) val = 0.5 * val;
+                    if (j == Ncy - 1) val = 0.5 * val;
+                    if (i == 0) val = 0.5 * val;
+                    if (i == Ncx - 1) val = 0.5 * val;
+                    Bint += val;
+                }
+        Bint = Bint * 8.0 * S->dV;
+        error = fabs(Bint + S->noe[at]) / fabs((double)S->noe[at]);
+        Rcut = Rcut + 1.0;
+    } while (Rcut <= 15.0 && error > S->tol_psc);
+    S->cutoff[at][0] = Ncx * S->delta[0];
+    S->cutoff[at][1] = Ncy * S->delta[1];
+    S->cutoff[at][2] = Ncz * S->delta[2];
+#undef VPS
+#undef BJA
+    free(Vps); free(BJ);
+    return SQO_OK;
+}
+
+/* Check_Overlap, lsdft_initObjs.cc:2612-2663 */
+static int check_overlap(int xcor, int ycor, int zcor, int lxdim, int lydim, int lzdim, int xs, int ys, int zs,
+                         int xl, int yl, int zl, int *xstart, int *ystart, int *zstart, int *xend, int *yend, int *zend)
+{
+    *xstart = *ystart = *zstart = *xend = *yend = *zend = -1000;
+    if (xs >= xcor && xs <= xcor + lxdim - 1) *xstart = xs;
+    else if (xcor >= xs && xcor <= xl) *xstart = xcor;
+    if (xl >= xcor && xl <= xcor + lxdim - 1) *xend = xl;
+    else if (xcor + lxdim - 1 >= xs && xcor + lxdim - 1 <= xl) *xend = xcor + lxdim - 1;
+    if (ys >= ycor && ys <= ycor + lydim - 1) *ystart = ys;
+    else if (ycor >= ys && ycor <= yl) *ystart = ycor;
+    if (yl >= ycor && yl <= ycor + lydim - 1) *yend = yl;
+    else if (ycor + lydim - 1 >= ys && ycor + lydim - 1 <= yl) *yend = ycor + lydim - 1;
+    if (zs >= zcor && zs <= zcor + lzdim - 1) *zstart = zs;
+    else if (zcor >= zs && zcor <= zl) *zstart = zcor;
+    if (zl >= zcor && zl <= zcor + lzdim - 1) *zend = zl;
+    else if (zcor + lzdim - 1 >= zs && zcor + lzdim - 1 <= zl) *zend = zcor + lzdim - 1;
+    return (*xstart != -1000) && (*xend != -1000) && (*ystart != -1000) && (*yend != -1000) &&
+           (*zstart != -1000) && (*zend != -1000);
+}
+
+/* Atom position -> nearest grid index as the reference computes it:
+ * roundf((x0 + R)/h) rounds a float (lsdft_initObjs.cc:1534-1536, 1800-1802). */
+static int grid_round(double v) { return (int)roundf((float)v); }
+
+/* ------------------------------------------------------------- public API */
+
+void sqo_free(sqo_sys *S);
+
+/* Initialize (lsdft_initObjs.cc:36-93) minus the PETSc objects: Read_Parameters,
+ * Read_Ion, Read_Pseudopotential, Calculate_SplineFit,
+ * Calculate_NonlocalCoefficients, Calculate_PseudochargeCutoff, and
+ * Create_SupercellAtomList + Calculate_NonlocalIndex with no phantom atom.
+ * Reads <dir>/<name>.inpt, <dir>/<name>.ion, <dir>/pseudopotential/<file>. */
+int sqo_load(const char *dir, const char *name, sqo_sys **out)
+{
+    char path[1024];
+    int rc;
+    sqo_sys *S = (sqo_sys *)calloc(1, sizeof(sqo_sys));
+    if (!S) return SQO_ERR_ALLOC;
+    snprintf(path, sizeof path, "%s/%s.inpt", dir, name);
+    if ((rc = read_inpt(S, path))) goto fail;
+    snprintf(path, sizeof path, "%s/%s.ion", dir, name);
+    if ((rc = read_ion(S, path))) goto fail;
+    for (int at = 0; at < S->ntype; at++) {
+        snprintf(path, sizeof path, "%s/pseudopotential/%s", dir, S->psdName[at]);
+        if ((rc = read_pot(&S->psd[at], path, S->atomType[at], S->localPsd[at]))) goto fail;
+        if ((rc = spline_fit(&S->psd[at]))) goto fail;
+        if ((rc = nonlocal_gamma(S, &S->psd[at], S->localPsd[at]))) goto fail;
+        if ((rc = pseudocharge_cutoff(S, at))) goto fail;
+    }
+    extern int sqo_set_phantom(sqo_sys *, int, double, double, double);
+    if ((rc = sqo_set_phantom(S, 0, 0, 0, 0))) goto fail;
+    *out = S;
+    return SQO_OK;
+fail:
+    sqo_free(S);
+    *out = NULL;
+    return rc;
+}
+
+/* Create_SupercellAtomList (lsdft_initObjs.cc:3031-3041) + Calculate_NonlocalIndex
+ * (875-967).  The reference sets endPosSC[0] = 3*nAtoms, one past the last
+ * coordinate, so every atom loop visits a phantom atom whose coordinates are
+ * read past the end of the position array (undefined behaviour).  enable=0
+ * drops it (the loops visit the real atoms only); enable=1 places it at
+ * (x,y,z), as if the memory after the array held those values.  Only Ntype=1
+ * reaches the phantom: the reference leaves startPosSC/endPosSC of other types
+ * unset, and here they take the .ion ranges. */
+int sqo_set_phantom(sqo_sys *S, int enable, double x, double y, double z)
+{
+    free(S->atomposSC);
+    S->atomposSC = dalloc(3 * (size_t)S->nAtoms + 3);
+    memcpy(S->atomposSC, S->atompos, sizeof(double) * 3 * S->nAtoms);
+    S->atomposSC[3 * S->nAtoms] = x;
+    S->atomposSC[3 * S->nAtoms + 1] = y;
+    S->atomposSC[3 * S->nAtoms + 2] = z;
+    for (int at = 0; at < S->ntype; at++) {
+        S->startPosSC[at] = S->startPos[at];
+        S->endPosSC[at] = S->endPos[at];
+    }
+    if (enable) {
+        if (S->ntype != 1) return SQO_ERR_ARG;
+        S->startPosSC[0] = 0;
+        S->endPosSC[0] = 3 * S->nAtoms;
+    }
+    /* Calculate_NonlocalIndex: IP_index[p] = first alpha slot of atom p */
+    int natoms_sc = 0, len = 0;
+    for (int at = 0; at < S->ntype; at++) {
+        int start = S->startPosSC[at] / 3, end = S->endPosSC[at] / 3;
+        natoms_sc += end - start + 1;
+        int lmax = S->psd[at].lmax;
+        for (int p = start; p <= end; p++)
+            for (int l = 0; l <= lmax; l++)
+                if (l != S->localPsd[at]) len += 2 * l + 1;
+    }
+    free(S->IP_index);
+    S->IP_index = ialloc(natoms_sc);
+    S->nIP = natoms_sc;
+    S->IP_length = len;
+    int index = 0, posIdx = 0;
+    for (int at = 0; at < S->ntype; at++) {
+        int start = S->startPosSC[at] / 3, end = S->endPosSC[at] / 3;
+        for (int p = start; p <= end; p++) {
+            S->IP_index[index++] = posIdx;
+            for (int l = 0; l <= S->psd[at].lmax; l++)
+                if (l != S->localPsd[at]) posIdx += 2 * l + 1;
+        }
+    }
+    S->have_nl = 0;
+    return SQO_OK;
+}
+
+/* Image range of the periodic map (BC=1), lsdft_initObjs.cc:1490-1499 / 1763-1770 */
+static void image_range(const sqo_sys *S, int at, int Imax[3])
+{
+    for (int d = 0; d < 3; d++) Imax[d] = (int)ceil(S->cutoff[at][d] / S->range[d]);
+}
+
+/* GetInfluencingAtoms_Local count, lsdft_initObjs.cc:1409-1665: number of atom
+ * images whose pseudocharge box overlaps the rank box (xcor,ycor,zcor,lx,ly,lz). */
+int sqo_count_local_overlaps(sqo_sys *S, const int box[6], int *counts)
+{
+    int index = 0;
+    for (int at = 0; at < S->ntype; at++) {
+        int off[3], Imax[3], count = 0;
+        for (int d = 0; d < 3; d++) off[d] = (int)ceil(S->cutoff[at][d] / S->delta[d] + 0.5);
+        image_range(S, at, Imax);
+        int start = S->startPosSC[at] / 3, end = S->endPosSC[at] / 3;
+        for (int p = start; p <= end; p++) {
+            double X0 = S->atomposSC[index++], Y0 = S->atomposSC[index++], Z0 = S->atomposSC[index++];
+            for (int PP = -Imax[0]; PP <= Imax[0]; PP++)
+                for (int QQ = -Imax[1]; QQ <= Imax[1]; QQ++)
+                    for (int RR = -Imax[2]; RR <= Imax[2]; RR++) {
+                        double x0 = X0 + PP * 2.0 * S->range[0];
+                        double y0 = Y0 + QQ * 2.0 * S->range[1];
+                        double z0 = Z0 + RR * 2.0 * S->range[2];
+                        int xi = grid_round((x0 + S->range[0]) / S->delta[0]);
+                        int yj = grid_round((y0 + S->range[1]) / S->delta[1]);
+                        int zk = grid_round((z0 + S->range[2]) / S->delta[2]);
+                        int xs, ys, zs, xe, ye, ze;
+                        count += check_overlap(box[0], box[1], box[2], box[3], box[4], box[5], xi - off[0],
+                                               yj - off[1], zk - off[2], xi + off[0], yj + off[1], zk + off[2],
+                                               &xs, &ys, &zs, &xe, &ye, &ze);
+                    }
+        }
+        counts[at] = count;
+    }
+    return SQO_OK;
+}
+
+static void free_nl(sqo_sys *S)
+{
+    for (int at = 0; at < SQO_MAXTYPE; at++) {
+        sqo_overlap *O = &S->ovl[at];
+        for (int p = 0; p < O->n; p++) {
+            if (O->chi) free(O->chi[p]);
+            if (O->lId) free(O->lId[p]);
+            if (O->mId) free(O->mId[p]);
+            if (O->kId) free(O->kId[p]);
+            if (O->jId) free(O->jId[p]);
+            if (O->iId) free(O->iId[p]);
+        }
+        free(O->chi); free(O->lId); free(O->mId); free(O->kId); free(O->jId); free(O->iId); free(O->ndim);
+        free(O->X0); free(O->Y0); free(O->Z0); free(O->xs); free(O->ys); free(O->zs);
+        free(O->xe); free(O->ye); free(O->ze); free(O->xindex);
+        memset(O, 0, sizeof *O);
+    }
+    S->have_nl = 0;
+}
+
+/* GetInfluencingAtoms_Nonlocal (lsdft_initObjs.cc:1671-1946) for the rank box
+ * box = (xcor,ycor,zcor,lx,ly,lz) extended by ceil(Rcut/h) on every side, then
+ * CalculateNonlocalProjectors (lsdft_nl.cc:141-208) and the flattening of
+ * NonlocalProjectors (lsdft_nl.cc:6-130): k,j,i outer, (l,m) inner. */
+int sqo_build_nonlocal(sqo_sys *S, const int box[6])
+{
+    free_nl(S);
+    memcpy(S->box, box, sizeof(int) * 6);
+    int nr[3];
+    for (int d = 0; d < 3; d++) nr[d] = (int)ceil(S->rcut / S->delta[d]);
+    int xcor = box[0] - nr[0], ycor = box[1] - nr[1], zcor = box[2] - nr[2];
+    int lxdim = box[3] + 2 * nr[0], lydim = box[4] + 2 * nr[1], lzdim = box[5] + 2 * nr[2];
+    int index = 0;
+    for (int at = 0; at < S->ntype; at++) {
+        sqo_psd *P = &S->psd[at];
+        sqo_overlap *O = &S->ovl[at];
+        double max1 = P->rc[0] > P->rc[1] ? P->rc[0] : P->rc[1];
+        double max2 = P->rc[2] > P->rc[3] ? P->rc[2] : P->rc[3];
+        double cut = max1 > max2 ? max1 : max2;
+        int off[3], Imax[3];
+        for (int d = 0; d < 3; d++) off[d] = (int)ceil(cut / S->delta[d]);
+        image_range(S, at, Imax);
+        int start = S->startPosSC[at] / 3, end = S->endPosSC[at] / 3;
+        /* the reference counts first, then stores; one pass with a growing list is equivalent */
+        int cap = 64;
+        O->X0 = dalloc(cap); O->Y0 = dalloc(cap); O->Z0 = dalloc(cap);
+        O->xs = ialloc(cap); O->ys = ialloc(cap); O->zs = ialloc(cap);
+        O->xe = ialloc(cap); O->ye = ialloc(cap); O->ze = ialloc(cap); O->xindex = ialloc(cap);
+        for (int p = start; p <= end; p++) {
+            double X0 = S->atomposSC[index++], Y0 = S->atomposSC[index++], Z0 = S->atomposSC[index++];
+            for (int PP = -Imax[0]; PP <= Imax[0]; PP++)
+                for (int QQ = -Imax[1]; QQ <= Imax[1]; QQ++)
+                    for (int RR = -Imax[2]; RR <= Imax[2]; RR++) {
+                        if (P->lmax == 0) continue;
+                        double x0 = X0 + PP * 2.0 * S->range[0];
+                        double y0 = Y0 + QQ * 2.0 * S->range[1];
+                        double z0 = Z0 + RR * 2.0 * S->range[2];
+                        int xi = grid_round((x0 + S->range[0] + 0 * S->rcut) / S->delta[0]);
+                        int yj = grid_round((y0 + S->range[1] + 0 * S->rcut) / S->delta[1]);
+                        int zk = grid_round((z0 + S->range[2] + 0 * S->rcut) / S->delta[2]);
+                        int xs, ys, zs, xe, ye, ze;
+                        if (!check_overlap(xcor, ycor, zcor, lxdim, lydim, lzdim, xi - off[0], yj - off[1],
+                                           zk - off[2], xi + off[0], yj + off[1], zk + off[2], &xs, &ys, &zs,
+                                           &xe, &ye, &ze))
+                            continue;
+                        if (O->n == cap) {
+                            cap *= 2;
+                            O->X0 = realloc(O->X0, cap * sizeof(double)); O->Y0 = realloc(O->Y0, cap * sizeof(double));
+                            O->Z0 = realloc(O->Z0, cap * sizeof(double));
+                            O->xs = realloc(O->xs, cap * sizeof(int)); O->ys = realloc(O->ys, cap * sizeof(int));
+                            O->zs = realloc(O->zs, cap * sizeof(int)); O->xe = realloc(O->xe, cap * sizeof(int));
+                            O->ye = realloc(O->ye, cap * sizeof(int)); O->ze = realloc(O->ze, cap * sizeof(int));
+                            O->xindex = realloc(O->xindex, cap * sizeof(int));
+                        }
+                        int c = O->n++;
+                        O->X0[c] = x0; O->Y0[c] = y0; O->Z0[c] = z0;
+                        O->xs[c] = xs; O->ys[c] = ys; O->zs[c] = zs;
+                        O->xe[c] = xe; O->ye[c] = ye; O->ze[c] = ze;
+                        O->xindex[c] = index - 3;
+                    }
+        }
+        /* projectors */
+        int lloc = S->localPsd[at];
+        int Nlm = P->lmax * P->lmax + 2 * (P->lmax - lloc);
+        O->ndim = ialloc(O->n);
+        O->chi = calloc(O->n ? O->n : 1, sizeof(double *));
+        O->lId = calloc(O->n ? O->n : 1, sizeof(int *)); O->mId = calloc(O->n ? O->n : 1, sizeof(int *));
+        O->kId = calloc(O->n ? O->n : 1, sizeof(int *)); O->jId = calloc(O->n ? O->n : 1, sizeof(int *));
+        O->iId = calloc(O->n ? O->n : 1, sizeof(int *));
+        for (int c = 0; c < O->n; c++) {
+            int nz = O->ze[c] - O->zs[c] + 1, ny = O->ye[c] - O->ys[c] + 1, nx = O->xe[c] - O->xs[c] + 1;
+            int nd = Nlm * nz * ny * nx;
+            O->ndim[c] = nd;
+            O->chi[c] = dalloc(nd);
+            O->lId[c] = ialloc(nd); O->mId[c] = ialloc(nd); O->kId[c] = ialloc(nd);
+            O->jId[c] = ialloc(nd); O->iId[c] = ialloc(nd);
+            int LI = 0;
+            for (int k = O->zs[c]; k <= O->ze[c]; k++) {
+                double z = S->delta[2] * k - S->range[2] - O->Z0[c];
+                double zz = z * z;
+                for (int j = O->ys[c]; j <= O->ye[c]; j++) {
+                    double y = S->delta[1] * j - S->range[1] - O->Y0[c];
+                    double yy = y * y;
+                    for (int i = O->xs[c]; i <= O->xe[c]; i++) {
+                        double x = S->delta[0] * i - S->range[0] - O->X0[c];
+                        double r = sqrt(x * x + yy + zz);
+                        for (int l = 0; l <= P->lmax; l++) {
+                            if (l == lloc) continue;
+                            double ul;
+                            if (sqo_spline_eval(P->r, P->UDeltaV[l], P->size, r, P->YD_UDeltaV[l], &ul)) {
+                                free_nl(S); return SQO_ERR_RANGE;
+                            }
+                            for (int m = -l; m <= l; m++) {
+                                int e2;
+                                double sh = sqo_ylm(x, y, z, l, m, &e2);
+                                if (e2) { free_nl(S); return e2; }
+                                O->chi[c][LI] = ul * sh;
+                                O->lId[c][LI] = l; O->mId[c][LI] = m;
+                                O->kId[c][LI] = k; O->jId[c][LI] = j; O->iId[c][LI] = i;
+                                LI++;
+                            }
+                        }
+                    }
+                }
+            }
+        }
+    }
+    S->have_nl = 1;
+    return SQO_OK;
+}
+/* Note on the projector coordinate: the reference writes
+ * x = delta_x*(i + xstart) - range_x - X0 with i counted from xstart
+ * (lsdft_nl.cc:189-190); the loop above counts the global index directly, the
+ * same integer, so x is bit-identical. */
+
+/* Create_LaplacianLocal (lsdft_initObjs.cc:393-475): the 6*order+1 point star of
+ * -(1/2) Lap_h on the n_x*n_y*n_z cube with zero-Dirichlet truncation
+ * (DM_BOUNDARY_GHOSTED drops columns outside the cube).  Stored as AIJ rows
+ * with ascending column index, the order SeqAIJ keeps. */
+static int build_cube_laplacian(sqo_sys *S)
+{
+    if (S->rowptr) return SQO_OK;
+    int nx = S->nloc[0], ny = S->nloc[1], nz = S->nloc[2], o = S->order;
+    int nc = nx * ny * nz;
+    S->nc = nc;
+    S->rowptr = ialloc(nc + 1);
+    S->colidx = ialloc((size_t)nc * (6 * o + 1));
+    S->val = dalloc((size_t)nc * (6 * o + 1));
+    int nnz = 0;
+    double cdiag = S->coeffs[0][0] + S->coeffs[1][0] + S->coeffs[2][0];
+    for (int k = 0; k < nz; k++)
+        for (int j = 0; j < ny; j++)
+            for (int i = 0; i < nx; i++) {
+                int row = (k * ny + j) * nx + i;
+                S->rowptr[row] = nnz;
+#define PUT(kk, jj, ii, v) do { S->colidx[nnz] = ((kk) * ny + (jj)) * nx + (ii); S->val[nnz] = (v); nnz++; } while (0)
+                for (int l = o; l >= 1; l--) if (k - l >= 0) PUT(k - l, j, i, S->coeffs[2][l]);
+                for (int l = o; l >= 1; l--) if (j - l >= 0) PUT(k, j - l, i, S->coeffs[1][l]);
+                for (int l = o; l >= 1; l--) if (i - l >= 0) PUT(k, j, i - l, S->coeffs[0][l]);
+                PUT(k, j, i, cdiag);
+                for (int l = 1; l <= o; l++) if (i + l < nx) PUT(k, j, i + l, S->coeffs[0][l]);
+                for (int l = 1; l <= o; l++) if (j + l < ny) PUT(k, j + l, i, S->coeffs[1][l]);
+                for (int l = 1; l <= o; l++) if (k + l < nz) PUT(k + l, j, i, S->coeffs[2][l]);
+#undef PUT
+            }
+    S->rowptr[nc] = nnz;
+    S->center = (((nz - 1) / 2) * ny + (ny - 1) / 2) * nx + (nx - 1) / 2; /* Nmid, lsdft_scf.cc:53-57 */
+    return SQO_OK;
+}
+
+/* per-thread scratch of one node's Lanczos */
+typedef struct {
+    double *dval;   /* diagonal-updated copy of the AIJ values (MatCopy + MatDiagonalSet) */
+    double *vk, *vkm1, *vkp1, *alpha, *veff;
+} node_ws;
+
+static int ws_init(node_ws *W, const sqo_sys *S)
+{
+    size_t nnz = S->rowptr[S->nc];
+    W->dval = dalloc(nnz);
+    W->vk = dalloc(S->nc); W->vkm1 = dalloc(S->nc); W->vkp1 = dalloc(S->nc); W->veff = dalloc(S->nc);
+    W->alpha = dalloc(S->IP_length > 0 ? S->IP_length : 1);
+    return (W->dval && W->vk && W->vkm1 && W->vkp1 && W->alpha && W->veff) ? SQO_OK : SQO_ERR_ALLOC;
+}
+static void ws_free(node_ws *W)
+{
+    free(W->dval); free(W->vk); free(W->vkm1); free(W->vkp1); free(W->alpha); free(W->veff);
+}
+
+/* V_eff on the node's cube: the global field (natural order, i fastest) wrapped
+ * periodically in grid index; NULL means the constant veff_const everywhere,
+ * which is what the reference uses (VecSet(VeffNodal, 0.333), lsdft_scf.cc:71). */
+static void cube_veff(const sqo_sys *S, const double *field, double veff_const, int Kp, int Jp, int Ip, double *out)
+{
+    int nx = S->nloc[0], ny = S->nloc[1], nz = S->nloc[2];
+    if (!field) { for (int p = 0; p < S->nc; p++) out[p] = veff_const; return; }
+    int Nx = S->npts[0], Ny = S->npts[1], Nz = S->npts[2];
+    int x0 = Ip - (nx - 1) / 2, y0 = Jp - (ny - 1) / 2, z0 = Kp - (nz - 1) / 2;
+    for (int k = 0; k < nz; k++) {
+        int K = ((z0 + k) % Nz + Nz) % Nz;
+        for (int j = 0; j < ny; j++) {
+            int J = ((y0 + j) % Ny + Ny) % Ny;
+            for (int i = 0; i < nx; i++) {
+                int I = ((x0 + i) % Nx + Nx) % Nx;
+                out[(k * ny + j) * nx + i] = field[((size_t)K * Ny + J) * Nx + I];
+            }
+        }
+    }
+}
+
+/* MatCopy(LaplacianOprloc) + MatDiagonalSet(+VeffNodal, ADD_VALUES), lsdft_scf.cc:73-75 */
+static void rebuild_operator(const sqo_sys *S, node_ws *W)
+{
+    memcpy(W->dval, S->val, sizeof(double) * S->rowptr[S->nc]);
+    for (int row = 0; row < S->nc; row++)
+        for (int q = S->rowptr[row]; q < S->rowptr[row + 1]; q++)
+            if (S->colidx[q] == row) { W->dval[q] = W->dval[q] + W->veff[row]; break; }
+}
+
+/* y = A x for the AIJ cube operator (PETSc MatMult_SeqAIJ row sum, taken as
+ * the sequential sum over ascending columns; PETSc's unrolling is unpinned). */
+static void spmv(const sqo_sys *S, const double *dval, const double *x, double *y)
+{
+    for (int row = 0; row < S->nc; row++) {
+        double sum = 0.0;
+        for (int q = S->rowptr[row]; q < S->rowptr[row + 1]; q++) sum += dval[q] * x[S->colidx[q]];
+        y[row] = sum;
+    }
+}
+
+/* V2 += V_nl V1 on the cube of node (Kp,Jp,Ip).
+ * mode 0 = Mult_NonlocalVector (lsdft_matvec.cc:25-149): every entry of atom p
+ *          uses the single slot alpha[IP_index[p]] (all (l,m) channels and all
+ *          images of the atom collapse into one coefficient).
+ * mode 1 = Mult_HamiltonianVector (lsdft_matvec.cc:155-293): slot
+ *          IP_index[p] + (l,m) channel, the Kleinman-Bylander operator.
+ * Entries are scanned over every image of the rank box, with the cube test of
+ * lsdft_matvec.cc:95,134. */
+static void nonlocal_apply(const sqo_sys *S, int mode, double *alpha, int Kp, int Jp, int Ip,
+                           const double *V1, double *V2)
+{
+    int nx = S->nloc[0], ny = S->nloc[1];
+    int hx = (int)ceil(S->rcut / S->delta[0]), hy = (int)ceil(S->rcut / S->delta[1]), hz = (int)ceil(S->rcut / S->delta[2]);
+    int Xstart = Ip - hx, Xend = Ip + hx, Ystart = Jp - hy, Yend = Jp + hy, Zstart = Kp - hz, Zend = Kp + hz;
+    memset(alpha, 0, sizeof(double) * (S->IP_length > 0 ? S->IP_length : 1));
+    for (int pass = 0; pass < 2; pass++) {
+        for (int at = 0; at < S->ntype; at++) {
+            const sqo_overlap *O = &S->ovl[at];
+            const double *G = S->psd[at].Gamma;
+            int lloc = S->localPsd[at];
+            for (int c = 0; c < O->n; c++) {
+                int p = O->xindex[c] / 3;
+                int base = S->IP_index[p];
+                const double *chi = O->chi[c];
+                const int *lI = O->lId[c], *mI = O->mId[c], *kI = O->kId[c], *jI = O->jId[c], *iI = O->iId[c];
+                for (int LI = 0; LI < O->ndim[c]; LI++) {
+                    int k = kI[LI], j = jI[LI], i = iI[LI];
+                    if ((k <= Zend && k >= Zstart) && (j <= Yend && j >= Ystart) && (i <= Xend && i >= Xstart)) {
+                        int slot = base;
+                        if (mode == 1) {
+                            int l = lI[LI], ch = 0;
+                            for (int ll = 0; ll < l; ll++) if (ll != lloc) ch += 2 * ll + 1;
+                            slot = base + ch + (mI[LI] + l);
+                        }
+                        size_t q = ((size_t)(k - Zstart) * ny + (j - Ystart)) * nx + (i - Xstart);
+                        if (pass == 0) alpha[slot] += V1[q] * chi[LI] * G[lI[LI]];
+                        else V2[q] += chi[LI] * alpha[slot];
+                    }
+                }
+            }
+        }
+    }
+}
+
+/* One H application on node (Kp,Jp,Ip): y = (L + diag V_eff) x + V_nl x. */
+int sqo_apply_h(sqo_sys *S, int mode, const double *veff_field, double veff_const, int Kp, int Jp, int Ip,
+                const double *x, double *y)
+{
+    if (!S->have_nl || (mode != 0 && mode != 1)) return SQO_ERR_ARG;
+    build_cube_laplacian(S);
+    node_ws W;
+    if (ws_init(&W, S)) { ws_free(&W); return SQO_ERR_ALLOC; }
+    cube_veff(S, veff_field, veff_const, Kp, Jp, Ip, W.veff);
+    rebuild_operator(S, &W);
+    spmv(S, W.dval, x, y);
+    nonlocal_apply(S, mode, W.alpha, Kp, Jp, Ip, x, y);
+    ws_free(&W);
+    return SQO_OK;
+}
+
+static double vdot(const double *x, const double *y, int n)
+{
+    double s = 0.0;
+    for (int i = 0; i < n; i++) s += y[i] * x[i];
+    return s;
+}
+static double vnorm(const double *x, int n)
+{
+    double s = 0.0;
+    for (int i = 0; i < n; i++) s += x[i] * x[i];
+    return sqrt(s);
+}
+
+/* CalculateSpectralNodesAndWeights, lsdft_scf.cc:87-164 (Lanczos part).
+ * a[0..nqp], b[0..nqp] are written (a[nqp], b[nqp] are not used by the QL). */
+static int lanczos_node(const sqo_sys *S, int mode, node_ws *W, int Kp, int Jp, int Ip, double *a, double *b)
+{
+    int n = S->nc, N_qp = S->nqp;
+    memset(W->vk, 0, sizeof(double) * n);
+    memset(W->vkm1, 0, sizeof(double) * n);
+    memset(W->vkp1, 0, sizeof(double) * n);
+    W->vkm1[S->center] = 1.0;
+    spmv(S, W->dval, W->vkm1, W->vk);
+    nonlocal_apply(S, mode, W->alpha, Kp, Jp, Ip, W->vkm1, W->vk);
+    a[0] = vdot(W->vkm1, W->vk, n);
+    for (int i = 0; i < n; i++) W->vk[i] = W->vk[i] + (-a[0]) * W->vkm1[i];
+    b[0] = vnorm(W->vk, n);
+    if (b[0] == 0.0) return SQO_ERR_BREAKDOWN;
+    double s = 1.0 / b[0];
+    for (int i = 0; i < n; i++) W->vk[i] = s * W->vk[i];
+    for (int k = 0; k < N_qp; k++) {
+        spmv(S, W->dval, W->vk, W->vkp1);
+        nonlocal_apply(S, mode, W->alpha, Kp, Jp, Ip, W->vk, W->vkp1);
+        a[k + 1] = vdot(W->vk, W->vkp1, n);
+        for (int i = 0; i < n; i++) W->vkp1[i] = W->vkp1[i] + (-a[k + 1]) * W->vk[i];
+        for (int i = 0; i < n; i++) W->vkp1[i] = W->vkp1[i] + (-b[k]) * W->vkm1[i];
+        memcpy(W->vkm1, W->vk, sizeof(double) * n);
+        b[k + 1] = vnorm(W->vkp1, n);
+        memcpy(W->vk, W->vkp1, sizeof(double) * n);
+        if (b[k + 1] == 0.0) return SQO_ERR_BREAKDOWN;
+        s = 1.0 / b[k + 1];
+        for (int i = 0; i < n; i++) W->vk[i] = s * W->vk[i];
+    }
+    return SQO_OK;
+}
+
+/* Lanczos coefficients a[0..nqp], b[0..nqp] of one node */
+int sqo_lanczos(sqo_sys *S, int mode, const double *veff_field, double veff_const, int Kp, int Jp, int Ip,
+                double *a, double *b)
+{
+    if (!S->have_nl || (mode != 0 && mode != 1)) return SQO_ERR_ARG;
+    build_cube_laplacian(S);
+    node_ws W;
+    if (ws_init(&W, S)) { ws_free(&W); return SQO_ERR_ALLOC; }
+    cube_veff(S, veff_field, veff_const, Kp, Jp, Ip, W.veff);
+    rebuild_operator(S, &W);
+    int rc = lanczos_node(S, mode, &W, Kp, Jp, Ip, a, b);
+    ws_free(&W);
+    return rc;
+}
+
+/* NodesAndWeights_LSSGQ, lsdft_scf.cc:32-82, for the nodes of the rank box
+ * given to sqo_build_nonlocal.  node_ids (or NULL for all, in LIpos order
+ * (k-zcor)*lx*ly + (j-ycor)*lx + (i-xcor)) selects which nodes to evaluate;
+ * row r of lam/w (nqp entries each) and lmin/lmax belong to node_ids[r].
+ * nthreads > 1 evaluates nodes in parallel (one scratch set per thread). */
+int sqo_nodes_and_weights(sqo_sys *S, int mode, const double *veff_field, double veff_const,
+                          const int *node_ids, int n_nodes, double *lam, double *w, double *lmin, double *lmax,
+                          int nthreads)
+{
+    if (!S->have_nl || (mode != 0 && mode != 1)) return SQO_ERR_ARG;
+    build_cube_laplacian(S);
+    int lx = S->box[3], ly = S->box[4], lz = S->box[5];
+    int total = lx * ly * lz;
+    if (!node_ids) n_nodes = total;
+    int err = SQO_OK;
+    if (nthreads < 1) nthreads = 1;
+#pragma omp parallel num_threads(nthreads)
+    {
+        node_ws W;
+        int e = ws_init(&W, S);
+        double *a = dalloc(S->nqp + 1), *b = dalloc(S->nqp + 1);
+#pragma omp for schedule(dynamic, 1)
+        for (int r = 0; r < n_nodes; r++) {
+            if (e) continue;
+            int id = node_ids ? node_ids[r] : r;
+            if (id < 0 || id >= total) { e = SQO_ERR_ARG; continue; }
+            int i = S->box[0] + id % lx, j = S->box[1] + (id / lx) % ly, k = S->box[2] + id / (lx * ly);
+            cube_veff(S, veff_field, veff_const, k, j, i, W.veff);
+            rebuild_operator(S, &W);
+            int rc = lanczos_node(S, mode, &W, k, j, i, a, b);
+            if (!rc) rc = sqo_tridiag_nodes_weights(a, b, S->nqp, lam + (size_t)r * S->nqp, w + (size_t)r * S->nqp,
+                                                    lmin + r, lmax + r);
+            if (rc) e = rc;
+        }
+        if (e) {
+#pragma omp critical
+            err = e;
+        }
+        free(a); free(b);
+        ws_free(&W);
+    }
+    return err;
+}
+
+/* ------------------------------------------------ density, Fermi, energy
+ * The reference declares these only (lsdft.h:98-102, 209).  Written from SQ
+ * theory: with the unit vector e_p, rho_p = (2/dV) sum_i w_pi f(lambda_pi),
+ * the Fermi level solves 2 sum_p sum_i w_pi f(lambda_pi) = N_e, the band energy
+ * is 2 sum w lambda f and -TS = (2/beta) sum w [f ln f + (1-f) ln(1-f)],
+ * f(l) = 1/(1+exp(beta (l - lambda_f))).  The root is found by bisection on
+ * [min lambda_min - 1, max lambda_max + 1] to a bracket of 1e-14 (relative)
+ * or 200 halvings. */
+static double fermi(double x) /* 1/(1+e^x) */
+{
+    if (x > 0) { double t = exp(-x); return t / (1.0 + t); }
+    return 1.0 / (1.0 + exp(x));
+}
+/* f ln f + (1-f) ln(1-f) at x = beta (lambda - lambda_f):
+ * = -log1p(t) - |x| t/(1+t), t = exp(-|x|). */
+static double fermi_entropy_term(double x)
+{
+    double ax = fabs(x), t = exp(-ax);
+    return -log1p(t) - ax * t / (1.0 + t);
+}
+
+double sqo_electron_count(const double *lam, const double *w, long nq, double beta, double lf)
+{
+    double s = 0.0;
+    for (long q = 0; q < nq; q++) s += w[q] * fermi(beta * (lam[q] - lf));
+    return 2.0 * s;
+}
+
+int sqo_density_fermi_energy(const double *lam, const double *w, int n_nodes, int nqp, double beta, double nelec,
+                             double dV, double *lambda_f, double *rho, double *eband, double *mts)
+{
+    long nq = (long)n_nodes * nqp;
+    if (nq <= 0) return SQO_ERR_ARG;
+    double lo = lam[0], hi = lam[0];
+    for (long q = 0; q < nq; q++) { if (lam[q] < lo) lo = lam[q]; if (lam[q] > hi) hi = lam[q]; }
+    lo -= 1.0; hi += 1.0;
+    double glo = sqo_electron_count(lam, w, nq, beta, lo) - nelec;
+    double ghi = sqo_electron_count(lam, w, nq, beta, hi) - nelec;
+    if (glo > 0 || ghi < 0) return SQO_ERR_RANGE;
+    for (int it = 0; it < 200; it++) {
+        double mid = 0.5 * (lo + hi);
+        if (hi - lo <= 1e-14 * (fabs(mid) > 1.0 ? fabs(mid) : 1.0)) break;
+        double g = sqo_electron_count(lam, w, nq, beta, mid) - nelec;
+        if (g > 0) hi = mid; else lo = mid;
+    }
+    double lf = 0.5 * (lo + hi);
+    *lambda_f = lf;
+    double eb = 0.0, ts = 0.0;
+    for (int p = 0; p < n_nodes; p++) {
+        double s = 0.0;
+        for (int i = 0; i < nqp; i++) {
+            long q = (long)p * nqp + i;
+            double x = beta * (lam[q] - lf);
+            double f = fermi(x);
+            s += w[q] * f;
+            eb += w[q] * lam[q] * f;
+            ts += w[q] * fermi_entropy_term(x);
+        }
+        if (rho) rho[p] = 2.0 * s / dV;
+    }
+    *eband = 2.0 * eb;
+    *mts = 2.0 * ts / beta;
+    return SQO_OK;
+}
+
+/* ------------------------------------------------------------- accessors */
+typedef struct {
+    int npts[3], nloc[3], order, nqp, ntype, nAtoms, noetot, IP_length, nIP, nc, center, bc;
+    double range[3], delta[3], dV, beta, rcut;
+    double coeffs[3][SQO_MAXORDER + 1];
+    double cutoff[SQO_MAXTYPE][3];
+    double gamma[SQO_MAXTYPE][4], rc[SQO_MAXTYPE][4];
+    int lmax[SQO_MAXTYPE], lloc[SQO_MAXTYPE], psd_size[SQO_MAXTYPE], nimages[SQO_MAXTYPE];
+} sqo_info_t;
+
+void sqo_info(sqo_sys *S, sqo_info_t *I)
+{
+    memset(I, 0, sizeof *I);
+    build_cube_laplacian(S);
+    for (int d = 0; d < 3; d++) {
+        I->npts[d] = S->npts[d]; I->nloc[d] = S->nloc[d]; I->range[d] = S->range[d]; I->delta[d] = S->delta[d];
+        for (int p = 0; p <= S->order; p++) I->coeffs[d][p] = S->coeffs[d][p];
+    }
+    I->order = S->order; I->nqp = S->nqp; I->ntype = S->ntype; I->nAtoms = S->nAtoms; I->noetot = S->noetot;
+    I->IP_length = S->IP_length; I->nIP = S->nIP; I->nc = S->nc; I->center = S->center; I->bc = S->bc;
+    I->dV = S->dV; I->beta = S->beta; I->rcut = S->rcut;
+    for (int at = 0; at < S->ntype; at++) {
+        for (int d = 0; d < 3; d++) I->cutoff[at][d] = S->cutoff[at][d];
+        for (int l = 0; l < 4; l++) { I->gamma[at][l] = S->psd[at].Gamma[l]; I->rc[at][l] = S->psd[at].rc[l]; }
+        I->lmax[at] = S->psd[at].lmax; I->lloc[at] = S->localPsd[at]; I->psd_size[at] = S->psd[at].size;
+        I->nimages[at] = S->have_nl ? S->ovl[at].n : -1;
+    }
+}
+
+int sqo_info_size(void) { return (int)sizeof(sqo_info_t); }
+
+/* Radial tables of type at: r, Vloc, and UDeltaV/U/YD for channel l (any may be NULL). */
+int sqo_get_radial(sqo_sys *S, int at, int l, double *r, double *vloc, double *udv, double *yd_udv, double *u,
+                   double *yd_u)
+{
+    if (at < 0 || at >= S->ntype || l < 0 || l > S->psd[at].lmax) return SQO_ERR_ARG;
+    sqo_psd *P = &S->psd[at];
+    size_t n = sizeof(double) * P->size;
+    if (r) memcpy(r, P->r, n);
+    if (vloc) memcpy(vloc, P->Vloc, n);
+    if (udv) memcpy(udv, P->UDeltaV[l], n);
+    if (yd_udv) memcpy(yd_udv, P->YD_UDeltaV[l], n);
+    if (u) memcpy(u, P->U[l], n);
+    if (yd_u) memcpy(yd_u, P->YD_U[l], n);
+    return SQO_OK;
+}
+
+/* Image c of type at: center (3), box (xs,ys,zs,xe,ye,ze), atom index p, ndim */
+int sqo_get_image(sqo_sys *S, int at, int c, double *center, int *box, int *atom, int *ndim)
+{
+    if (!S->have_nl || at < 0 || at >= S->ntype || c < 0 || c >= S->ovl[at].n) return SQO_ERR_ARG;
+    sqo_overlap *O = &S->ovl[at];
+    center[0] = O->X0[c]; center[1] = O->Y0[c]; center[2] = O->Z0[c];
+    box[0] = O->xs[c]; box[1] = O->ys[c]; box[2] = O->zs[c];
+    box[3] = O->xe[c]; box[4] = O->ye[c]; box[5] = O->ze[c];
+    *atom = O->xindex[c] / 3;
+    *ndim = O->ndim[c];
+    return SQO_OK;
+}
+
+int sqo_get_chi(sqo_sys *S, int at, int c, double *chi, int *lId, int *mId, int *kId, int *jId, int *iId)
+{
+    if (!S->have_nl || at < 0 || at >= S->ntype || c < 0 || c >= S->ovl[at].n) return SQO_ERR_ARG;
+    sqo_overlap *O = &S->ovl[at];
+    size_t n = O->ndim[c];
+    if (chi) memcpy(chi, O->chi[c], n * sizeof(double));
+    if (lId) memcpy(lId, O->lId[c], n * sizeof(int));
+    if (mId) memcpy(mId, O->mId[c], n * sizeof(int));
+    if (kId) memcpy(kId, O->kId[c], n * sizeof(int));
+    if (jId) memcpy(jId, O->jId[c], n * sizeof(int));
+    if (iId) memcpy(iId, O->iId[c], n * sizeof(int));
+    return SQO_OK;
+}
+
+int sqo_get_ip_index(sqo_sys *S, int *out) { memcpy(out, S->IP_index, sizeof(int) * S->nIP); return S->nIP; }
+
+/* Atom positions of the .ion file (3*nAtoms) */
+int sqo_get_atoms(sqo_sys *S, double *pos) { memcpy(pos, S->atompos, sizeof(double) * 3 * S->nAtoms); return S->nAtoms; }
+
+/* Dense cube Laplacian values (CSR) for brute-force checks. */
+int sqo_get_cube_csr(sqo_sys *S, int *rowptr, int *col, double *val)
+{
+    build_cube_laplacian(S);
+    int nnz = S->rowptr[S->nc];
+    if (rowptr) memcpy(rowptr, S->rowptr, sizeof(int) * (S->nc + 1));
+    if (col) memcpy(col, S->colidx, sizeof(int) * nnz);
+    if (val) memcpy(val, S->val, sizeof(double) * nnz);
+    return nnz;
+}
+
+/* Override the Lanczos degree (LANCZOS_DEGREE) after loading. */
+void sqo_set_nqp(sqo_sys *S, int nqp) { S->nqp = nqp; }
+
+void sqo_free(sqo_sys *S)
+{
+    if (!S) return;
+    free_nl(S);
+    free(S->atompos); free(S->atomposSC); free(S->IP_index);
+    free(S->rowptr); free(S->colidx); free(S->val);
+    for (int at = 0; at < SQO_MAXTYPE; at++) {
+        sqo_psd *P = &S->psd[at];
+        free(P->r); free(P->uu); free(P->Vloc); free(P->YD_Vloc); free(P->YD_uu);
+        for (int l = 0; l < 4; l++) { free(P->V[l]); free(P->U[l]); free(P->UDeltaV[l]); free(P->YD_UDeltaV[l]); free(P->YD_U[l]); }
+    }
+    free(S);
+}

@@ -1,0 +1,225 @@
+"""ctypes binding of libsqamd.so (include/sq_amd.h).
+
+The library must be built in-tree (python -m sqminiapp_amd.build); loading
+fails loudly when it is missing -- there is no fallback path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libsqamd.so")
+
+SQ_NL_REF_COLLAPSED = 0
+SQ_NL_KB = 1
+SQ_FP64 = 0
+SQ_FP32 = 1
+
+STATUS = {0: "SQ_OK", 1: "SQ_ERR_IO", 2: "SQ_ERR_FORMAT", 3: "SQ_ERR_RANGE", 4: "SQ_ERR_ARG", 5: "SQ_ERR_QL",
+          6: "SQ_ERR_ALLOC", 7: "SQ_ERR_BREAKDOWN", 8: "SQ_ERR_DEVICE", 9: "SQ_ERR_STATE"}
+
+# every symbol include/sq_amd.h declares
+EXPORTS = ["sq_last_error", "sq_default_options", "sq_create_from_deck", "sq_destroy", "sq_get_info", "sq_set_veff",
+           "sq_run", "sq_get_nodes_and_weights", "sq_get_lanczos", "sq_nodes_and_weights", "sq_last_timing",
+           "sq_apply_h", "sq_tridiag_nodes_weights", "sq_density_fermi_energy", "sq_get_image"]
+
+
+class SQError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{STATUS.get(code, code)}: {msg}")
+        self.code = code
+
+
+class Options(C.Structure):
+    _fields_ = [("phantom_atom", C.c_int), ("phantom_pos", C.c_double * 3), ("rank_box", C.c_int * 6),
+                ("nqp_override", C.c_int), ("device", C.c_int)]
+
+
+class Info(C.Structure):
+    _fields_ = [
+        ("npts", C.c_int * 3), ("nloc", C.c_int * 3), ("halfwidth", C.c_int * 3), ("order", C.c_int),
+        ("nqp", C.c_int), ("ntype", C.c_int), ("nAtoms", C.c_int), ("noetot", C.c_int),
+        ("n_images", C.c_int), ("n_sc_atoms", C.c_int), ("ip_length", C.c_int), ("n_nodes", C.c_int),
+        ("nc", C.c_int), ("rank_box", C.c_int * 6), ("ext_origin", C.c_int * 3), ("ext_dims", C.c_int * 3),
+        ("range", C.c_double * 3), ("delta", C.c_double * 3), ("dV", C.c_double), ("beta", C.c_double),
+        ("rcut", C.c_double), ("coeffs", (C.c_double * 11) * 3), ("gamma", (C.c_double * 4) * 8),
+        ("cutoff", (C.c_double * 3) * 8), ("chi_entries", C.c_longlong),
+    ]
+
+
+_lib = None
+
+
+def lib():
+    """Load libsqamd.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m sqminiapp_amd.build`")
+    L = C.CDLL(LIB_PATH)
+    dp, ip, vp, lg = C.POINTER(C.c_double), C.POINTER(C.c_int), C.c_void_p, C.c_long
+    L.sq_last_error.restype = C.c_char_p
+    L.sq_default_options.argtypes = [C.POINTER(Options)]
+    L.sq_create_from_deck.argtypes = [C.c_char_p, C.c_char_p, C.POINTER(Options), C.POINTER(vp)]
+    L.sq_destroy.argtypes = [vp]
+    L.sq_get_info.argtypes = [vp, C.POINTER(Info)]
+    L.sq_set_veff.argtypes = [vp, dp, C.c_double]
+    L.sq_run.argtypes = [vp, C.c_int, C.c_int, lg, lg]
+    L.sq_get_nodes_and_weights.argtypes = [vp, dp, dp, dp, dp]
+    L.sq_get_lanczos.argtypes = [vp, dp, dp]
+    L.sq_nodes_and_weights.argtypes = [vp, C.c_int, C.c_int, lg, lg, dp, dp, dp, dp]
+    L.sq_last_timing.argtypes = [vp, dp, dp]
+    L.sq_apply_h.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, dp, dp]
+    L.sq_tridiag_nodes_weights.argtypes = [vp, dp, dp, C.c_int, lg, dp, dp, dp, dp]
+    L.sq_density_fermi_energy.argtypes = [vp, C.c_double, C.c_double, dp, dp, dp, dp]
+    L.sq_get_image.argtypes = [vp, C.c_int, dp, ip, ip, ip, ip, dp]
+    _lib = L
+    return L
+
+
+def _dp(a):
+    return None if a is None else a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _check(rc):
+    if rc != 0:
+        raise SQError(rc, lib().sq_last_error().decode())
+
+
+class Context:
+    """One SQ problem on one device (or host-only with device=-1)."""
+
+    def __init__(self, directory, name, *, phantom_atom=False, phantom_pos=(0.0, 0.0, 0.0), rank_box=None,
+                 nqp=0, device=0):
+        L = lib()
+        opt = Options()
+        L.sq_default_options(C.byref(opt))
+        opt.phantom_atom = int(bool(phantom_atom))
+        for d in range(3):
+            opt.phantom_pos[d] = float(phantom_pos[d])
+        if rank_box is not None:
+            for d in range(6):
+                opt.rank_box[d] = int(rank_box[d])
+        opt.nqp_override = int(nqp)
+        opt.device = int(device)
+        self._h = C.c_void_p()
+        _check(L.sq_create_from_deck(str(directory).encode(), str(name).encode(), C.byref(opt), C.byref(self._h)))
+        self.info = self._info()
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().sq_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _info(self):
+        I = Info()
+        _check(lib().sq_get_info(self._h, C.byref(I)))
+        return I
+
+    @property
+    def nqp(self):
+        return self.info.nqp
+
+    @property
+    def n_nodes(self):
+        return self.info.n_nodes
+
+    def set_veff(self, veff):
+        if veff is None or np.isscalar(veff):
+            _check(lib().sq_set_veff(self._h, None, 0.333 if veff is None else float(veff)))
+        else:
+            f = np.ascontiguousarray(veff, np.float64).ravel()
+            I = self.info
+            if f.size != I.npts[0] * I.npts[1] * I.npts[2]:
+                raise ValueError("V_eff field size differs from the grid")
+            _check(lib().sq_set_veff(self._h, _dp(f), 0.0))
+
+    def run(self, mode=SQ_NL_REF_COLLAPSED, precision=SQ_FP64, begin=0, count=None):
+        count = self.n_nodes - begin if count is None else count
+        _check(lib().sq_run(self._h, int(mode), int(precision), int(begin), int(count)))
+        self._last = (begin, count)
+
+    def results(self):
+        begin, count = self._last
+        n = self.nqp
+        lam = np.zeros((count, n))
+        w = np.zeros((count, n))
+        lmin = np.zeros(count)
+        lmax = np.zeros(count)
+        _check(lib().sq_get_nodes_and_weights(self._h, _dp(lam), _dp(w), _dp(lmin), _dp(lmax)))
+        return lam, w, lmin, lmax
+
+    def lanczos(self):
+        begin, count = self._last
+        a = np.zeros((count, self.nqp + 1))
+        b = np.zeros((count, self.nqp + 1))
+        _check(lib().sq_get_lanczos(self._h, _dp(a), _dp(b)))
+        return a, b
+
+    def nodes_and_weights(self, mode=SQ_NL_REF_COLLAPSED, precision=SQ_FP64, begin=0, count=None):
+        self.run(mode, precision, begin, count)
+        return self.results()
+
+    def timing(self):
+        t1, t2 = C.c_double(), C.c_double()
+        _check(lib().sq_last_timing(self._h, C.byref(t1), C.byref(t2)))
+        return t1.value, t2.value
+
+    def apply_h(self, node, x, mode=SQ_NL_REF_COLLAPSED):
+        K, J, I_ = node
+        x = np.ascontiguousarray(x, np.float64)
+        if x.size != self.info.nc:
+            raise ValueError("vector size differs from the cube")
+        y = np.zeros_like(x)
+        _check(lib().sq_apply_h(self._h, int(mode), int(K), int(J), int(I_), _dp(x), _dp(y)))
+        return y
+
+    def tridiag_nodes_weights(self, diag, subdiag):
+        diag = np.ascontiguousarray(diag, np.float64)
+        count, n = diag.shape
+        sub = np.zeros((count, n))
+        s = np.asarray(subdiag, np.float64)
+        sub[:, : min(n, s.shape[1])] = s[:, : min(n, s.shape[1])]
+        lam = np.zeros((count, n))
+        w = np.zeros((count, n))
+        lmin = np.zeros(count)
+        lmax = np.zeros(count)
+        _check(lib().sq_tridiag_nodes_weights(self._h, _dp(diag), _dp(sub), n, count, _dp(lam), _dp(w), _dp(lmin),
+                                              _dp(lmax)))
+        return lam, w, lmin, lmax
+
+    def density_fermi_energy(self, nelec=0.0, beta=0.0, want_rho=True):
+        lf, eb, mts = C.c_double(), C.c_double(), C.c_double()
+        rho = np.zeros(self.n_nodes) if want_rho else None
+        _check(lib().sq_density_fermi_energy(self._h, float(nelec), float(beta), C.byref(lf), _dp(rho), C.byref(eb),
+                                             C.byref(mts)))
+        return lf.value, rho, eb.value, mts.value
+
+    def image(self, c, with_chi=True):
+        cen = np.zeros(3)
+        box = np.zeros(6, np.int32)
+        atom, typ, ent = C.c_int(), C.c_int(), C.c_int()
+        ipt = C.POINTER(C.c_int)
+        _check(lib().sq_get_image(self._h, int(c), _dp(cen), box.ctypes.data_as(ipt), C.byref(atom), C.byref(typ),
+                                  C.byref(ent), None))
+        chi = None
+        if with_chi:
+            chi = np.zeros(ent.value)
+            _check(lib().sq_get_image(self._h, int(c), None, None, None, None, None, _dp(chi)))
+        return dict(center=cen, box=box, atom=atom.value, type=typ.value, entries=ent.value, chi=chi)

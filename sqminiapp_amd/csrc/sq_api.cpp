@@ -1,0 +1,626 @@
+// sq_api.cpp -- the C ABI of include/sq_amd.h: context creation (host setup +
+// upload), the batched nodes-and-weights run, single-node H application,
+// batched QL, and the density / Fermi level / energy reduction.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/sq_amd.h"
+#include "sq_kernels.h"
+#include "sq_system.h"
+
+using namespace sqamd;
+
+namespace {
+thread_local std::string g_err;
+
+int set_err(int code, const std::string &msg)
+{
+    g_err = msg;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                                     \
+    do {                                                                                                  \
+        hipError_t e_ = (expr);                                                                           \
+        if (e_ != hipSuccess)                                                                             \
+            return set_err(SQ_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_));             \
+    } while (0)
+
+constexpr int kRedBlocks = 512;
+}  // namespace
+
+struct sq_ctx {
+    System sys;
+    int device = 0, n_cu = 0, grid = 0, img_cap = 0, nlm_tmpl = 6;
+    size_t lds = 0;
+    int lds_A = 0, lds_alpha = 0, lds_red = 0, lds_local = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    // device tables
+    double *chi64 = nullptr;
+    float *chi32 = nullptr;
+    int4 *img_lo = nullptr, *img_dim = nullptr;
+    int64_t *img_off = nullptr;
+    int *img_atom = nullptr, *img_type = nullptr;
+    double *gamma = nullptr;
+    int *cov_off = nullptr, *cov_img = nullptr;
+    double *diag64 = nullptr;
+    float *diag32 = nullptr;
+    double *c64 = nullptr;  // 3 x (kMaxOrder+1)
+    float *c32 = nullptr;
+    void *scratch = nullptr;
+    double *a = nullptr, *b = nullptr, *lam = nullptr, *w = nullptr, *lmin = nullptr, *lmax = nullptr, *qlws = nullptr;
+    int *status = nullptr;
+    double *partial = nullptr, *partial2 = nullptr, *out = nullptr;
+    // last run
+    long last_begin = 0, last_count = 0;
+    bool have_run = false;
+    double t_lanczos = 0, t_ql = 0;
+};
+
+namespace {
+
+template <typename T>
+int dev_upload(T **dst, const T *src, size_t n)
+{
+    HIP_TRY(hipMalloc((void **)dst, std::max<size_t>(n, 1) * sizeof(T)));
+    if (n) HIP_TRY(hipMemcpy(*dst, src, n * sizeof(T), hipMemcpyHostToDevice));
+    return SQ_OK;
+}
+
+void free_ctx(sq_ctx *c)
+{
+    if (!c) return;
+    void *ptrs[] = {c->chi64, c->chi32, c->img_lo, c->img_dim, c->img_off, c->img_atom, c->img_type, c->gamma,
+                    c->cov_off, c->cov_img, c->diag64, c->diag32, c->c64, c->c32, c->scratch, c->a, c->b,
+                    c->lam, c->w, c->lmin, c->lmax, c->qlws, c->status, c->partial, c->partial2, c->out};
+    for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+    for (hipEvent_t e : c->ev)
+        if (e) (void)hipEventDestroy(e);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+// Largest number of images any cube of the rank box meets: a 3-D difference
+// array over nodes, one box per image expanded by the cube half-width.
+int max_images_per_cube(const System &S)
+{
+    const int lx = S.box[3], ly = S.box[4], lz = S.box[5];
+    std::vector<int> diff((size_t)(lx + 1) * (ly + 1) * (lz + 1), 0);
+    auto at = [&](int i, int j, int k) -> int & { return diff[((size_t)k * (ly + 1) + j) * (lx + 1) + i]; };
+    for (const Image &im : S.images) {
+        int lo[3], hi[3];
+        bool empty = false;
+        for (int d = 0; d < 3; d++) {
+            lo[d] = std::max(im.lo[d] - S.hw[d], S.box[d]) - S.box[d];
+            hi[d] = std::min(im.hi[d] + S.hw[d], S.box[d] + S.box[3 + d] - 1) - S.box[d];
+            if (hi[d] < lo[d]) empty = true;
+        }
+        if (empty) continue;
+        for (int s = 0; s < 8; s++) {
+            int i = (s & 1) ? hi[0] + 1 : lo[0], j = (s & 2) ? hi[1] + 1 : lo[1], k = (s & 4) ? hi[2] + 1 : lo[2];
+            int sign = ((s & 1) ? -1 : 1) * ((s & 2) ? -1 : 1) * ((s & 4) ? -1 : 1);
+            at(i, j, k) += sign;
+        }
+    }
+    for (int k = 0; k <= lz; k++)
+        for (int j = 0; j <= ly; j++)
+            for (int i = 1; i <= lx; i++) at(i, j, k) += at(i - 1, j, k);
+    for (int k = 0; k <= lz; k++)
+        for (int j = 1; j <= ly; j++)
+            for (int i = 0; i <= lx; i++) at(i, j, k) += at(i, j - 1, k);
+    for (int k = 1; k <= lz; k++)
+        for (int j = 0; j <= ly; j++)
+            for (int i = 0; i <= lx; i++) at(i, j, k) += at(i, j, k - 1);
+    int mx = 0;
+    for (int k = 0; k < lz; k++)
+        for (int j = 0; j < ly; j++)
+            for (int i = 0; i < lx; i++) mx = std::max(mx, at(i, j, k));
+    return mx;
+}
+
+int upload_diag(sq_ctx *c, const double *field, double veff_const)
+{
+    const System &S = c->sys;
+    const int64_t ne = S.ext_points();
+    std::vector<double> d64(ne);
+    std::vector<float> d32(ne);
+    for (int k = 0; k < S.extn[2]; k++) {
+        int K = (((S.ext0[2] + k) % S.npts[2]) + S.npts[2]) % S.npts[2];
+        for (int j = 0; j < S.extn[1]; j++) {
+            int J = (((S.ext0[1] + j) % S.npts[1]) + S.npts[1]) % S.npts[1];
+            for (int i = 0; i < S.extn[0]; i++) {
+                int I = (((S.ext0[0] + i) % S.npts[0]) + S.npts[0]) % S.npts[0];
+                double v = field ? field[((size_t)K * S.npts[1] + J) * S.npts[0] + I] : veff_const;
+                size_t e = ((size_t)k * S.extn[1] + j) * S.extn[0] + i;
+                d64[e] = S.cdiag + v;  // MatDiagonalSet(ADD_VALUES): diag + V_eff
+                d32[e] = (float)d64[e];
+            }
+        }
+    }
+    if (!c->diag64) {
+        HIP_TRY(hipMalloc((void **)&c->diag64, ne * sizeof(double)));
+        HIP_TRY(hipMalloc((void **)&c->diag32, ne * sizeof(float)));
+    }
+    HIP_TRY(hipMemcpy(c->diag64, d64.data(), ne * sizeof(double), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(c->diag32, d32.data(), ne * sizeof(float), hipMemcpyHostToDevice));
+    return SQ_OK;
+}
+
+int create_device(sq_ctx *c, int device)
+{
+    System &S = c->sys;
+    c->device = device;
+    HIP_TRY(hipSetDevice(device));
+    hipDeviceProp_t prop;
+    HIP_TRY(hipGetDeviceProperties(&prop, device));
+    c->n_cu = prop.multiProcessorCount;
+    HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    for (auto &e : c->ev) HIP_TRY(hipEventCreate(&e));
+    // projector tables
+    const int M = (int)S.images.size();
+    std::vector<int4> lo(M), dim(M);
+    std::vector<int64_t> off(M);
+    std::vector<int> atom(M), type(M);
+    for (int g = 0; g < M; g++) {
+        const Image &im = S.images[g];
+        lo[g] = make_int4(im.lo[0], im.lo[1], im.lo[2], 0);
+        dim[g] = make_int4(im.dim(0), im.dim(1), im.dim(2), im.nlm);
+        off[g] = im.chi_off;
+        atom[g] = im.atom;
+        type[g] = im.type;
+    }
+    int rc;
+    if ((rc = dev_upload(&c->img_lo, lo.data(), M))) return rc;
+    if ((rc = dev_upload(&c->img_dim, dim.data(), M))) return rc;
+    if ((rc = dev_upload(&c->img_off, off.data(), M))) return rc;
+    if ((rc = dev_upload(&c->img_atom, atom.data(), M))) return rc;
+    if ((rc = dev_upload(&c->img_type, type.data(), M))) return rc;
+    if ((rc = dev_upload(&c->chi64, S.chi.data(), S.chi.size()))) return rc;
+    std::vector<float> chi32(S.chi.begin(), S.chi.end());
+    if ((rc = dev_upload(&c->chi32, chi32.data(), chi32.size()))) return rc;
+    std::vector<double> gam((size_t)kMaxTypesDev * kMaxChannels, 0.0);
+    for (int t = 0; t < S.ntype; t++)
+        for (int ch = 0; ch < S.psd[t].nlm(); ch++) gam[t * kMaxChannels + ch] = S.psd[t].gamma[S.psd[t].ch_l[ch]];
+    if ((rc = dev_upload(&c->gamma, gam.data(), gam.size()))) return rc;
+    if ((rc = dev_upload(&c->cov_off, S.cov_off.data(), S.cov_off.size()))) return rc;
+    if ((rc = dev_upload(&c->cov_img, S.cov_img.data(), S.cov_img.size()))) return rc;
+    std::vector<double> cf(3 * (kMaxOrder + 1), 0.0);
+    std::vector<float> cf32(3 * (kMaxOrder + 1), 0.f);
+    for (int d = 0; d < 3; d++)
+        for (int p = 0; p <= S.order; p++) {
+            cf[d * (kMaxOrder + 1) + p] = S.coeff[d][p];
+            cf32[d * (kMaxOrder + 1) + p] = (float)S.coeff[d][p];
+        }
+    if ((rc = dev_upload(&c->c64, cf.data(), cf.size()))) return rc;
+    if ((rc = dev_upload(&c->c32, cf32.data(), cf32.size()))) return rc;
+    if ((rc = upload_diag(c, nullptr, 0.333))) return rc;
+    // per-node outputs
+    const int64_t nn = S.n_nodes();
+    HIP_TRY(hipMalloc((void **)&c->a, nn * (S.nqp + 1) * sizeof(double)));
+    HIP_TRY(hipMalloc((void **)&c->b, nn * (S.nqp + 1) * sizeof(double)));
+    HIP_TRY(hipMalloc((void **)&c->lam, nn * S.nqp * sizeof(double)));
+    HIP_TRY(hipMalloc((void **)&c->w, nn * S.nqp * sizeof(double)));
+    HIP_TRY(hipMalloc((void **)&c->lmin, nn * sizeof(double)));
+    HIP_TRY(hipMalloc((void **)&c->lmax, nn * sizeof(double)));
+    HIP_TRY(hipMalloc((void **)&c->status, std::max<int64_t>(nn, 1) * sizeof(int)));
+    HIP_TRY(hipMalloc((void **)&c->qlws, 3 * nn * S.nqp * sizeof(double)));
+    HIP_TRY(hipMalloc((void **)&c->partial, kRedBlocks * sizeof(double)));
+    HIP_TRY(hipMalloc((void **)&c->partial2, kRedBlocks * sizeof(double)));
+    HIP_TRY(hipMalloc((void **)&c->out, 4 * sizeof(double)));
+    // LDS layout of the Lanczos workgroup
+    c->img_cap = std::max(1, max_images_per_cube(S));
+    c->nlm_tmpl = S.max_nlm <= 6 ? 6 : kMaxChannels;
+    if (S.max_nlm > kMaxChannels) return set_err(SQ_ERR_ARG, "more (l,m) channels than the kernels support");
+    auto align16 = [](size_t x) { return (x + 15) & ~size_t(15); };
+    size_t o = align16((size_t)c->img_cap * sizeof(int));
+    c->lds_A = (int)o;
+    o = align16(o + (size_t)c->img_cap * c->nlm_tmpl * sizeof(double));
+    c->lds_alpha = (int)o;
+    o = align16(o + (size_t)c->img_cap * c->nlm_tmpl * sizeof(double));
+    c->lds_red = (int)o;
+    o = align16(o + 32 * sizeof(double));
+    c->lds_local = (int)o;
+    o = align16(o + (size_t)std::max(M, 1) * sizeof(short));
+    c->lds = o;
+    if (c->lds > 150 * 1024) return set_err(SQ_ERR_ARG, "projector tables exceed the LDS budget of a workgroup");
+    if (c->img_cap > 32767) return set_err(SQ_ERR_ARG, "too many images per cube");
+    // one resident workgroup per CU slot; the grid strides over nodes
+    c->grid = c->n_cu * 2;
+    HIP_TRY(hipMalloc(&c->scratch, (size_t)c->grid * 3 * S.nc() * sizeof(double)));
+    return SQ_OK;
+}
+
+KParams make_params(const sq_ctx *c, int mode, int precision, long begin, long count)
+{
+    const System &S = c->sys;
+    KParams P;
+    memset(&P, 0, sizeof P);
+    for (int d = 0; d < 3; d++) {
+        P.nloc[d] = S.nloc[d];
+        P.hw[d] = S.hw[d];
+        P.box0[d] = S.box[d];
+        P.box_n[d] = S.box[3 + d];
+        P.ext0[d] = S.ext0[d];
+        P.extn[d] = S.extn[d];
+    }
+    P.nc = (int)S.nc();
+    P.center = (((S.nloc[2] - 1) / 2) * S.nloc[1] + (S.nloc[1] - 1) / 2) * S.nloc[0] + (S.nloc[0] - 1) / 2;
+    P.order = S.order;
+    P.node_begin = begin;
+    P.node_count = count;
+    P.nqp = S.nqp;
+    P.mode = mode;
+    P.nlm_max = c->nlm_tmpl;
+    P.n_images = (int)S.images.size();
+    P.img_cap = c->img_cap;
+    P.lds_A = c->lds_A;
+    P.lds_alpha = c->lds_alpha;
+    P.lds_red = c->lds_red;
+    P.lds_local = c->lds_local;
+    const int st = kMaxOrder + 1;
+    if (precision == SQ_FP64) {
+        P.cx_ptr = c->c64; P.cy_ptr = c->c64 + st; P.cz_ptr = c->c64 + 2 * st;
+        P.diag_ext = c->diag64;
+        P.chi = c->chi64;
+    } else {
+        P.cx_ptr = c->c32; P.cy_ptr = c->c32 + st; P.cz_ptr = c->c32 + 2 * st;
+        P.diag_ext = c->diag32;
+        P.chi = c->chi32;
+    }
+    P.img_lo = c->img_lo;
+    P.img_dim = c->img_dim;
+    P.img_off = c->img_off;
+    P.img_atom = c->img_atom;
+    P.img_type = c->img_type;
+    P.gamma_ch = c->gamma;
+    P.cov_off = c->cov_off;
+    P.cov_img = c->cov_img;
+    P.scratch = c->scratch;
+    P.out_a = c->a;
+    P.out_b = c->b;
+    P.status = c->status;
+    return P;
+}
+
+int check_status(sq_ctx *c, long begin, long count, const char *what)
+{
+    std::vector<int> st(count);
+    HIP_TRY(hipMemcpy(st.data(), c->status + begin, count * sizeof(int), hipMemcpyDeviceToHost));
+    for (long r = 0; r < count; r++)
+        if (st[r]) {
+            char msg[160];
+            snprintf(msg, sizeof msg, "%s: node %ld (rank-box id) status %d", what, begin + r, st[r]);
+            int code = st[r] == 7 ? SQ_ERR_BREAKDOWN : st[r] == 5 ? SQ_ERR_QL : SQ_ERR_ARG;
+            return set_err(code, msg);
+        }
+    return SQ_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *sq_last_error(void) { return g_err.c_str(); }
+
+void sq_default_options(sq_options *opt)
+{
+    memset(opt, 0, sizeof *opt);
+}
+
+int sq_create_from_deck(const char *dir, const char *name, const sq_options *opt, sq_ctx **out)
+{
+    g_err.clear();
+    if (!dir || !name || !out) return set_err(SQ_ERR_ARG, "null argument");
+    *out = nullptr;
+    sq_options defaults;
+    sq_default_options(&defaults);
+    if (!opt) opt = &defaults;
+    SetupOptions so;
+    so.phantom_atom = opt->phantom_atom != 0;
+    for (int d = 0; d < 3; d++) so.phantom_pos[d] = opt->phantom_pos[d];
+    for (int d = 0; d < 6; d++) so.rank_box[d] = opt->rank_box[d];
+    so.nqp_override = opt->nqp_override;
+    sq_ctx *c = new sq_ctx();
+    Status st = build_system(dir, name, so, c->sys);
+    if (!st.ok()) {
+        free_ctx(c);
+        return set_err(st.code, st.msg);
+    }
+    if (c->sys.nqp > kMaxNqp) {
+        free_ctx(c);
+        return set_err(SQ_ERR_ARG, "LANCZOS_DEGREE above the supported maximum");
+    }
+    int rc = opt->device >= 0 ? create_device(c, opt->device) : SQ_OK;
+    if (rc) {
+        std::string keep = g_err;
+        free_ctx(c);
+        g_err = keep;
+        return rc;
+    }
+    *out = c;
+    return SQ_OK;
+}
+
+int sq_destroy(sq_ctx *ctx)
+{
+    if (ctx) {
+        if (ctx->stream) (void)hipSetDevice(ctx->device);
+        free_ctx(ctx);
+    }
+    return SQ_OK;
+}
+
+int sq_get_info(sq_ctx *c, sq_info *I)
+{
+    if (!c || !I) return set_err(SQ_ERR_ARG, "null argument");
+    const System &S = c->sys;
+    memset(I, 0, sizeof *I);
+    for (int d = 0; d < 3; d++) {
+        I->npts[d] = S.npts[d];
+        I->nloc[d] = S.nloc[d];
+        I->halfwidth[d] = S.hw[d];
+        I->range[d] = S.range[d];
+        I->delta[d] = S.h[d];
+        I->ext_origin[d] = S.ext0[d];
+        I->ext_dims[d] = S.extn[d];
+        for (int p = 0; p <= S.order && p < 11; p++) I->coeffs[d][p] = S.coeff[d][p];
+    }
+    for (int d = 0; d < 6; d++) I->rank_box[d] = S.box[d];
+    I->order = S.order;
+    I->nqp = S.nqp;
+    I->ntype = S.ntype;
+    I->nAtoms = S.nAtoms;
+    I->noetot = S.noetot;
+    I->n_images = (int)S.images.size();
+    I->n_sc_atoms = (int)S.sctype.size();
+    I->ip_length = S.ip_length;
+    I->n_nodes = (int)S.n_nodes();
+    I->nc = (int)S.nc();
+    I->dV = S.dV;
+    I->beta = S.beta;
+    I->rcut = S.rcut;
+    for (int t = 0; t < S.ntype && t < 8; t++) {
+        for (int l = 0; l < 4; l++) I->gamma[t][l] = S.psd[t].gamma[l];
+        for (int d = 0; d < 3; d++) I->cutoff[t][d] = S.cutoff[t][d];
+    }
+    I->chi_entries = (long long)S.chi.size();
+    return SQ_OK;
+}
+
+int sq_set_veff(sq_ctx *c, const double *field, double veff_const)
+{
+    if (!c) return set_err(SQ_ERR_ARG, "null context");
+    if (!c->stream) return set_err(SQ_ERR_STATE, "host-only context (created with device < 0)");
+    HIP_TRY(hipSetDevice(c->device));
+    return upload_diag(c, field, veff_const);
+}
+
+int sq_run(sq_ctx *c, int mode, int precision, long begin, long count)
+{
+    if (!c) return set_err(SQ_ERR_ARG, "null context");
+    if (!c->stream) return set_err(SQ_ERR_STATE, "host-only context (created with device < 0)");
+    const System &S = c->sys;
+    if (mode != SQ_NL_REF_COLLAPSED && mode != SQ_NL_KB) return set_err(SQ_ERR_ARG, "unknown nonlocal mode");
+    if (precision != SQ_FP64 && precision != SQ_FP32) return set_err(SQ_ERR_ARG, "unknown precision");
+    if (begin < 0 || count <= 0 || begin + count > S.n_nodes()) return set_err(SQ_ERR_ARG, "node range outside the rank box");
+    HIP_TRY(hipSetDevice(c->device));
+    c->have_run = false;
+    KParams P = make_params(c, mode, precision, begin, count);
+    const int grid = (int)std::min<long>(c->grid, count);
+    HIP_TRY(hipEventRecord(c->ev[0], c->stream));
+    HIP_TRY(launch_lanczos(P, precision, grid, c->lds, c->stream));
+    HIP_TRY(hipEventRecord(c->ev[1], c->stream));
+    HIP_TRY(hipEventSynchronize(c->ev[1]));
+    int rc = check_status(c, begin, count, "Lanczos");
+    if (rc) return rc;
+    HIP_TRY(hipEventRecord(c->ev[2], c->stream));
+    HIP_TRY(launch_ql(c->a + begin * (S.nqp + 1), c->b + begin * (S.nqp + 1), S.nqp, S.nqp + 1, count,
+                      c->lam + begin * S.nqp, c->w + begin * S.nqp, c->lmin + begin, c->lmax + begin, c->status + begin,
+                      c->qlws, c->stream));
+    HIP_TRY(hipEventRecord(c->ev[3], c->stream));
+    HIP_TRY(hipEventSynchronize(c->ev[3]));
+    if ((rc = check_status(c, begin, count, "QL"))) return rc;
+    float ms = 0;
+    HIP_TRY(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
+    c->t_lanczos = ms;
+    HIP_TRY(hipEventElapsedTime(&ms, c->ev[2], c->ev[3]));
+    c->t_ql = ms;
+    c->last_begin = begin;
+    c->last_count = count;
+    c->have_run = true;
+    return SQ_OK;
+}
+
+int sq_get_nodes_and_weights(sq_ctx *c, double *lambda, double *w, double *lmin, double *lmax)
+{
+    if (!c) return set_err(SQ_ERR_ARG, "null context");
+    if (!c->have_run) return set_err(SQ_ERR_STATE, "no completed sq_run");
+    const int nqp = c->sys.nqp;
+    const long b = c->last_begin, n = c->last_count;
+    HIP_TRY(hipSetDevice(c->device));
+    if (lambda) HIP_TRY(hipMemcpy(lambda, c->lam + b * nqp, n * nqp * sizeof(double), hipMemcpyDeviceToHost));
+    if (w) HIP_TRY(hipMemcpy(w, c->w + b * nqp, n * nqp * sizeof(double), hipMemcpyDeviceToHost));
+    if (lmin) HIP_TRY(hipMemcpy(lmin, c->lmin + b, n * sizeof(double), hipMemcpyDeviceToHost));
+    if (lmax) HIP_TRY(hipMemcpy(lmax, c->lmax + b, n * sizeof(double), hipMemcpyDeviceToHost));
+    return SQ_OK;
+}
+
+int sq_get_lanczos(sq_ctx *c, double *a, double *b)
+{
+    if (!c) return set_err(SQ_ERR_ARG, "null context");
+    if (!c->have_run) return set_err(SQ_ERR_STATE, "no completed sq_run");
+    const int s = c->sys.nqp + 1;
+    const long b0 = c->last_begin, n = c->last_count;
+    HIP_TRY(hipSetDevice(c->device));
+    if (a) HIP_TRY(hipMemcpy(a, c->a + b0 * s, n * s * sizeof(double), hipMemcpyDeviceToHost));
+    if (b) HIP_TRY(hipMemcpy(b, c->b + b0 * s, n * s * sizeof(double), hipMemcpyDeviceToHost));
+    return SQ_OK;
+}
+
+int sq_nodes_and_weights(sq_ctx *c, int mode, int precision, long begin, long count, double *lambda, double *w,
+                         double *lmin, double *lmax)
+{
+    int rc = sq_run(c, mode, precision, begin, count);
+    if (rc) return rc;
+    return sq_get_nodes_and_weights(c, lambda, w, lmin, lmax);
+}
+
+int sq_last_timing(sq_ctx *c, double *lanczos_ms, double *ql_ms)
+{
+    if (!c) return set_err(SQ_ERR_ARG, "null context");
+    if (!c->have_run) return set_err(SQ_ERR_STATE, "no completed sq_run");
+    if (lanczos_ms) *lanczos_ms = c->t_lanczos;
+    if (ql_ms) *ql_ms = c->t_ql;
+    return SQ_OK;
+}
+
+int sq_apply_h(sq_ctx *c, int mode, int Kp, int Jp, int Ip, const double *x, double *y)
+{
+    if (!c || !x || !y) return set_err(SQ_ERR_ARG, "null argument");
+    if (!c->stream) return set_err(SQ_ERR_STATE, "host-only context (created with device < 0)");
+    const System &S = c->sys;
+    if (mode != SQ_NL_REF_COLLAPSED && mode != SQ_NL_KB) return set_err(SQ_ERR_ARG, "unknown nonlocal mode");
+    int rel[3] = {Ip - S.box[0], Jp - S.box[1], Kp - S.box[2]};
+    for (int d = 0; d < 3; d++)
+        if (rel[d] < 0 || rel[d] >= S.box[3 + d]) return set_err(SQ_ERR_ARG, "node outside the rank box");
+    const long id = ((long)rel[2] * S.box[4] + rel[1]) * S.box[3] + rel[0];
+    HIP_TRY(hipSetDevice(c->device));
+    const size_t nc = S.nc();
+    double *dx = nullptr, *dy = nullptr;
+    HIP_TRY(hipMalloc((void **)&dx, nc * sizeof(double)));
+    hipError_t e = hipMalloc((void **)&dy, nc * sizeof(double));
+    if (e != hipSuccess) { (void)hipFree(dx); return set_err(SQ_ERR_DEVICE, "hipMalloc failed"); }
+    int rc = SQ_OK;
+    KParams P = make_params(c, mode, SQ_FP64, id, 1);
+    int zero = 0;
+    if (hipMemcpy(dx, x, nc * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(c->status, &zero, sizeof(int), hipMemcpyHostToDevice) != hipSuccess ||
+        launch_apply(P, id, dx, dy, c->lds, c->stream) != hipSuccess || hipStreamSynchronize(c->stream) != hipSuccess ||
+        hipMemcpy(y, dy, nc * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
+        rc = set_err(SQ_ERR_DEVICE, "apply_h device step failed");
+    if (!rc) {
+        int st = 0;
+        (void)hipMemcpy(&st, c->status, sizeof(int), hipMemcpyDeviceToHost);
+        if (st) rc = set_err(SQ_ERR_ARG, "apply_h: image list overflow");
+    }
+    (void)hipFree(dx);
+    (void)hipFree(dy);
+    c->have_run = false;  // status slot 0 was reused
+    return rc;
+}
+
+int sq_tridiag_nodes_weights(sq_ctx *c, const double *diag, const double *subdiag, int n, long count, double *lambda,
+                             double *w, double *lmin, double *lmax)
+{
+    if (!c || !diag || !subdiag) return set_err(SQ_ERR_ARG, "null argument");
+    if (!c->stream) return set_err(SQ_ERR_STATE, "host-only context (created with device < 0)");
+    if (n < 1 || n > kMaxNqp || count <= 0) return set_err(SQ_ERR_ARG, "bad tridiagonal size");
+    HIP_TRY(hipSetDevice(c->device));
+    const size_t m = (size_t)n * count;
+    double *buf = nullptr;
+    int *st = nullptr;
+    HIP_TRY(hipMalloc((void **)&buf, (2 * m + 2 * m + 2 * count + 3 * m) * sizeof(double)));
+    hipError_t e = hipMalloc((void **)&st, count * sizeof(int));
+    if (e != hipSuccess) { (void)hipFree(buf); return set_err(SQ_ERR_DEVICE, "hipMalloc failed"); }
+    double *da = buf, *db = da + m, *dl = db + m, *dw = dl + m, *dmin = dw + m, *dmax = dmin + count, *ws = dmax + count;
+    int rc = SQ_OK;
+    if (hipMemcpy(da, diag, m * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(db, subdiag, m * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+        launch_ql(da, db, n, n, count, dl, dw, dmin, dmax, st, ws, c->stream) != hipSuccess ||
+        hipStreamSynchronize(c->stream) != hipSuccess)
+        rc = set_err(SQ_ERR_DEVICE, "QL device step failed");
+    if (!rc) {
+        std::vector<int> hs(count);
+        (void)hipMemcpy(hs.data(), st, count * sizeof(int), hipMemcpyDeviceToHost);
+        for (long r = 0; r < count && !rc; r++)
+            if (hs[r]) rc = set_err(SQ_ERR_QL, "QL: more than 50 sweeps for tridiagonal " + std::to_string(r));
+    }
+    if (!rc) {
+        if ((lambda && hipMemcpy(lambda, dl, m * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess) ||
+            (w && hipMemcpy(w, dw, m * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess) ||
+            (lmin && hipMemcpy(lmin, dmin, count * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess) ||
+            (lmax && hipMemcpy(lmax, dmax, count * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess))
+            rc = set_err(SQ_ERR_DEVICE, "QL copy-back failed");
+    }
+    (void)hipFree(buf);
+    (void)hipFree(st);
+    return rc;
+}
+
+int sq_density_fermi_energy(sq_ctx *c, double nelec, double beta, double *lambda_f, double *rho, double *e_band,
+                            double *minus_ts)
+{
+    if (!c || !lambda_f || !e_band || !minus_ts) return set_err(SQ_ERR_ARG, "null argument");
+    const System &S = c->sys;
+    const long nn = S.n_nodes();
+    if (!c->have_run || c->last_begin != 0 || c->last_count != nn)
+        return set_err(SQ_ERR_STATE, "density needs a completed sq_run over every node of the rank box");
+    if (nelec <= 0) nelec = S.noetot;
+    if (beta <= 0) beta = S.beta;
+    HIP_TRY(hipSetDevice(c->device));
+    const long nq = nn * S.nqp;
+    double range[2];
+    HIP_TRY(launch_lam_range(c->lmin, c->lmax, nn, c->out, c->stream));
+    HIP_TRY(hipMemcpyAsync(range, c->out, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    double lo = range[0] - 1.0, hi = range[1] + 1.0;
+    auto count_at = [&](double lf, double *res) -> int {
+        HIP_TRY(launch_occupation(c->lam, c->w, nq, beta, lf, c->partial, kRedBlocks, c->out, c->stream));
+        HIP_TRY(hipMemcpyAsync(res, c->out, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        return SQ_OK;
+    };
+    double glo, ghi;
+    int rc;
+    if ((rc = count_at(lo, &glo)) || (rc = count_at(hi, &ghi))) return rc;
+    if (glo - nelec > 0 || ghi - nelec < 0) return set_err(SQ_ERR_RANGE, "Fermi level not bracketed by the spectrum");
+    // bisection on [lambda_min - 1, lambda_max + 1] to 1e-14 (relative), at most 200 halvings
+    for (int it = 0; it < 200; it++) {
+        double mid = 0.5 * (lo + hi);
+        if (hi - lo <= 1e-14 * (std::fabs(mid) > 1.0 ? std::fabs(mid) : 1.0)) break;
+        double g;
+        if ((rc = count_at(mid, &g))) return rc;
+        if (g - nelec > 0) hi = mid;
+        else lo = mid;
+    }
+    const double lf = 0.5 * (lo + hi);
+    double res[2];
+    double *drho = nullptr;
+    if (rho) HIP_TRY(hipMalloc((void **)&drho, nn * sizeof(double)));
+    hipError_t e1 = launch_density_energy(c->lam, c->w, nn, S.nqp, beta, lf, 2.0 / S.dV, drho, c->partial, c->partial2,
+                                          kRedBlocks, c->out, c->stream);
+    hipError_t e2 = e1 == hipSuccess ? hipMemcpyAsync(res, c->out, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream) : e1;
+    hipError_t e3 = e2 == hipSuccess ? hipStreamSynchronize(c->stream) : e2;
+    if (e3 == hipSuccess && rho) e3 = hipMemcpy(rho, drho, nn * sizeof(double), hipMemcpyDeviceToHost);
+    if (drho) (void)hipFree(drho);
+    if (e3 != hipSuccess) return set_err(SQ_ERR_DEVICE, std::string("density step: ") + hipGetErrorString(e3));
+    *lambda_f = lf;
+    *e_band = res[0];
+    *minus_ts = res[1];
+    return SQ_OK;
+}
+
+int sq_get_image(sq_ctx *c, int g, double *center, int *box, int *atom, int *type, int *entries, double *chi)
+{
+    if (!c) return set_err(SQ_ERR_ARG, "null context");
+    const System &S = c->sys;
+    if (g < 0 || g >= (int)S.images.size()) return set_err(SQ_ERR_ARG, "image index out of range");
+    const Image &im = S.images[g];
+    if (center) { center[0] = im.X0; center[1] = im.Y0; center[2] = im.Z0; }
+    if (box) for (int d = 0; d < 3; d++) { box[d] = im.lo[d]; box[3 + d] = im.hi[d]; }
+    if (atom) *atom = im.atom;
+    if (type) *type = im.type;
+    if (entries) *entries = (int)(im.points() * im.nlm);
+    if (chi) memcpy(chi, S.chi.data() + im.chi_off, im.points() * im.nlm * sizeof(double));
+    return SQ_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,52 @@
+// sq_kernels.h -- launch parameters and launchers of the SQ device kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace sqamd {
+
+constexpr int kMaxChannels = 16;  // (l,m) channels per type for lmax <= 3
+constexpr int kMaxNqp = 256;      // Lanczos degree limit of the QL kernel
+constexpr int kMaxTypesDev = 8;
+
+// Everything a Lanczos / apply kernel reads; passed by value.
+struct KParams {
+    // cube
+    int nloc[3], nc, center, order, hw[3];
+    // rank box and its extension
+    int box0[3], box_n[3], ext0[3], extn[3];
+    // run
+    long node_begin, node_count;
+    int nqp, mode, nlm_max, n_images, img_cap;
+    // LDS layout (byte offsets)
+    int lds_A, lds_alpha, lds_red, lds_local;
+    // stencil weights cx[0..order] per axis (element type = precision)
+    const void *cx_ptr, *cy_ptr, *cz_ptr;
+    // cube diagonal fl(cdiag + V_eff) on the extended box (element type = precision)
+    const void *diag_ext;
+    // projector images: lo (x,y,z,-), dims (nx,ny,nz,nlm), chi offset, atom, type
+    const int4 *img_lo, *img_dim;
+    const int64_t *img_off;
+    const int *img_atom, *img_type;
+    const void *chi;             // element type = precision
+    const double *gamma_ch;      // [type][kMaxChannels]
+    // images covering each extended-box point, ascending
+    const int *cov_off, *cov_img;
+    // outputs and scratch
+    void *scratch;               // 3*nc elements per workgroup
+    double *out_a, *out_b;       // [node][nqp+1]
+    int *status;                 // [node]
+};
+
+hipError_t launch_lanczos(const KParams &P, int precision, int grid, size_t lds, hipStream_t st);
+hipError_t launch_apply(const KParams &P, long id, const double *x, double *y, size_t lds, hipStream_t st);
+hipError_t launch_ql(const double *a, const double *b, int n, int stride, long count, double *lam, double *w,
+                     double *lmin, double *lmax, int *status, double *ws, hipStream_t st);
+hipError_t launch_occupation(const double *lam, const double *w, long nq, double beta, double lf, double *partial,
+                             int nblocks, double *out, hipStream_t st);
+hipError_t launch_density_energy(const double *lam, const double *w, long n_nodes, int nqp, double beta, double lf,
+                                 double two_over_dv, double *rho, double *part_e, double *part_s, int nblocks,
+                                 double *out2, hipStream_t st);
+hipError_t launch_lam_range(const double *lmin, const double *lmax, long n, double *out, hipStream_t st);
+
+}  // namespace sqamd

@@ -1,0 +1,670 @@
+// sq_kernels.hip -- gfx950 kernels of the SQ hot path.
+//
+// One workgroup runs the whole Lanczos recurrence of one grid node's cube
+// (persistent over the N_qp+1 H applications) and then takes the next node;
+// the Lanczos vectors live in a per-workgroup slice of HBM that stays resident
+// in the L2/Infinity cache while the node is processed.  Per H application:
+//   gather  : A[img][ch] = sum_{cube & box} v * chi   (one wavefront per image,
+//             DPP/shuffle reduction, no atomics -> deterministic)
+//   alpha   : per atom, from the A of its images (collapsed or KB slots)
+//   apply   : u = sum over the 6*order+1 stencil columns in ascending column
+//             order, + diag(L)+V_eff, then + chi*alpha over the images covering
+//             the point in ascending image order (the reference's order)
+//   reduce  : a = <v,u>, then w = u - a v - b vprev, b = ||w||
+// This file is compiled with -ffp-contract=off: in the FP64 form every
+// product and sum rounds as written.  The FP32 form uses explicit fmaf.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "sq_kernels.h"
+
+namespace sqamd {
+
+constexpr int kWG = 1024;          // threads of the Lanczos workgroup
+constexpr int kWaves = kWG / 64;
+
+// Lanczos vectors are stored raw (w before scaling) with their scale factor
+// s = 1/b; v_i = s * w_i is formed at every read, which is the value VecScale
+// (x_i <- (1/b) x_i) stores in the reference (lsdft_scf.cc:131,147).
+
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v)
+{
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// Compensated (TwoSum) pair used by the FP32 form's dot products.
+struct CSum {
+    float hi, lo;
+};
+__device__ __forceinline__ CSum two_sum(float a, float b)
+{
+    float s = a + b;
+    float bb = s - a;
+    float err = (a - (s - bb)) + (b - bb);
+    return {s, err};
+}
+__device__ __forceinline__ CSum csum_add(CSum x, float y)
+{
+    CSum t = two_sum(x.hi, y);
+    t.lo += x.lo;
+    return t;
+}
+__device__ __forceinline__ CSum csum_merge(CSum x, CSum y)
+{
+    CSum t = two_sum(x.hi, y.hi);
+    t.lo += x.lo + y.lo;
+    return t;
+}
+__device__ __forceinline__ CSum wave_csum(CSum v)
+{
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        CSum o{__shfl_xor(v.hi, off, 64), __shfl_xor(v.lo, off, 64)};
+        v = csum_merge(v, o);
+    }
+    return v;
+}
+
+// Block-wide deterministic sum of one double per thread (fixed tree).
+__device__ double block_sum(double v, double *red)
+{
+    v = wave_sum(v);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) red[wave] = v;
+    __syncthreads();
+    double s = 0.0;
+    if (threadIdx.x == 0) {
+        for (int w = 0; w < kWaves; w++) s += red[w];
+        red[kWaves] = s;
+    }
+    __syncthreads();
+    s = red[kWaves];
+    __syncthreads();
+    return s;
+}
+
+// Block-wide compensated sum (FP32 form); result widened to double.
+__device__ double block_csum(CSum v, double *red)
+{
+    v = wave_csum(v);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) red[wave] = (double)v.hi + (double)v.lo;
+    __syncthreads();
+    double s = 0.0;
+    if (threadIdx.x == 0) {
+        for (int w = 0; w < kWaves; w++) s += red[w];
+        red[kWaves] = s;
+    }
+    __syncthreads();
+    s = red[kWaves];
+    __syncthreads();
+    return s;
+}
+
+// q / d for 0 <= q < 2^22, 1 <= d < 2^10 via a float reciprocal (exact after
+// the two corrections).
+__device__ __forceinline__ int fdiv(int q, int d, float inv)
+{
+    int r = (int)((float)q * inv);
+    if (r * d > q) r--;
+    if ((r + 1) * d <= q) r++;
+    return r;
+}
+
+// ------------------------------------------------------------------ gather
+// A[li][ch] = sum over the points of image li's box inside the cube of
+// v(pt) * chi_ch(pt); one wavefront per image.
+template <typename T, int NLM>
+__device__ void gather_images(const KParams &P, const int *img_list, int nimg, const int cube_lo[3], const T *vraw,
+                              T vs, const T *chi, double *A)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int nx = P.nloc[0], nxy = P.nloc[0] * P.nloc[1];
+    for (int li = wave; li < nimg; li += kWaves) {
+        const int g = img_list[li];
+        const int4 lo4 = P.img_lo[g];
+        const int4 dm4 = P.img_dim[g];
+        const int ilo[3] = {lo4.x, lo4.y, lo4.z};
+        const int idm[3] = {dm4.x, dm4.y, dm4.z};
+        int a[3], n[3];
+        for (int d = 0; d < 3; d++) {
+            int s = max(ilo[d], cube_lo[d]);
+            int e = min(ilo[d] + idm[d] - 1, cube_lo[d] + P.nloc[d] - 1);
+            a[d] = s;
+            n[d] = e - s + 1;
+        }
+        const int npt = n[0] * n[1] * n[2];
+        const float inv0 = 1.0f / (float)n[0], inv1 = 1.0f / (float)n[1];
+        const T *c = chi + P.img_off[g];
+        const int nlm = dm4.w;
+        T acc[NLM];
+#pragma unroll
+        for (int ch = 0; ch < NLM; ch++) acc[ch] = T(0);
+        for (int q = lane; q < npt; q += 64) {
+            int jk = fdiv(q, n[0], inv0);
+            int qi = q - jk * n[0];
+            int qk = fdiv(jk, n[1], inv1);
+            int qj = jk - qk * n[1];
+            int gx = a[0] + qi, gy = a[1] + qj, gz = a[2] + qk;
+            int p = (gz - cube_lo[2]) * nxy + (gy - cube_lo[1]) * nx + (gx - cube_lo[0]);
+            T v = vs * vraw[p];
+            int64_t b = ((int64_t)((gz - ilo[2]) * idm[1] + (gy - ilo[1])) * idm[0] + (gx - ilo[0])) * nlm;
+#pragma unroll
+            for (int ch = 0; ch < NLM; ch++)
+                if (ch < nlm) {
+                    if constexpr (sizeof(T) == 8) acc[ch] = acc[ch] + v * c[b + ch];
+                    else acc[ch] = fmaf(v, c[b + ch], acc[ch]);
+                }
+        }
+#pragma unroll
+        for (int ch = 0; ch < NLM; ch++) {
+            double s = wave_sum((double)acc[ch]);
+            if (lane == 0 && ch < nlm) A[li * NLM + ch] = s;
+        }
+    }
+}
+
+// alpha of every image in the node list from the A of all images of the same
+// atom (they are contiguous in the ascending list).
+template <int NLM>
+__device__ void images_alpha(const KParams &P, const int *img_list, int nimg, const double *A, double *alpha)
+{
+    for (int li = threadIdx.x; li < nimg; li += blockDim.x) {
+        const int g = img_list[li];
+        const int atom = P.img_atom[g];
+        const int type = P.img_type[g];
+        const int nlm = P.img_dim[g].w;
+        int s = li, e = li;
+        while (s > 0 && P.img_atom[img_list[s - 1]] == atom) s--;
+        while (e + 1 < nimg && P.img_atom[img_list[e + 1]] == atom) e++;
+        const double *gam = P.gamma_ch + type * kMaxChannels;
+        if (P.mode == 0) {  // one coefficient per atom
+            double al = 0.0;
+            for (int q = s; q <= e; q++)
+                for (int ch = 0; ch < nlm; ch++) al += gam[ch] * A[q * NLM + ch];
+            for (int ch = 0; ch < NLM; ch++) alpha[li * NLM + ch] = al;
+        } else {            // per (l,m) channel
+            for (int ch = 0; ch < NLM; ch++) {
+                double al = 0.0;
+                if (ch < nlm) {
+                    for (int q = s; q <= e; q++) al += A[q * NLM + ch];
+                    al = gam[ch] * al;
+                }
+                alpha[li * NLM + ch] = al;
+            }
+        }
+    }
+}
+
+// u(p) = stencil + diag + nonlocal scatter for cube point p.
+template <typename T, int NLM>
+__device__ __forceinline__ T apply_point(const KParams &P, int p, int li, int lj, int lk, const int cube_lo[3],
+                                         const T *vraw, T vs, const T *diag_ext, const T *chi, const short *local_of,
+                                         const double *alpha, int64_t E)
+{
+    const int nx = P.nloc[0], ny = P.nloc[1], nz = P.nloc[2], nxy = nx * ny, o = P.order;
+    T sum = T(0);
+    auto term = [&](T cf, int q) {
+        if constexpr (sizeof(T) == 8) sum = sum + cf * (vs * vraw[q]);
+        else sum = fmaf(cf, vs * vraw[q], sum);
+    };
+    const T *cx = reinterpret_cast<const T *>(P.cx_ptr), *cy = reinterpret_cast<const T *>(P.cy_ptr),
+            *cz = reinterpret_cast<const T *>(P.cz_ptr);
+    for (int l = o; l >= 1; l--) if (lk - l >= 0) term(cz[l], p - l * nxy);
+    for (int l = o; l >= 1; l--) if (lj - l >= 0) term(cy[l], p - l * nx);
+    for (int l = o; l >= 1; l--) if (li - l >= 0) term(cx[l], p - l);
+    term(diag_ext[E], p);
+    for (int l = 1; l <= o; l++) if (li + l < nx) term(cx[l], p + l);
+    for (int l = 1; l <= o; l++) if (lj + l < ny) term(cy[l], p + l * nx);
+    for (int l = 1; l <= o; l++) if (lk + l < nz) term(cz[l], p + l * nxy);
+    // nonlocal scatter, images in ascending order, channels ascending
+    const int e0 = P.cov_off[E], e1 = P.cov_off[E + 1];
+    const int gx = cube_lo[0] + li, gy = cube_lo[1] + lj, gz = cube_lo[2] + lk;
+    for (int e = e0; e < e1; e++) {
+        const int g = P.cov_img[e];
+        const int4 lo4 = P.img_lo[g];
+        const int4 dm4 = P.img_dim[g];
+        const int nlm = dm4.w;
+        const int64_t b = ((int64_t)((gz - lo4.z) * dm4.y + (gy - lo4.y)) * dm4.x + (gx - lo4.x)) * nlm;
+        const T *c = chi + P.img_off[g] + b;
+        const double *al = alpha + local_of[g] * NLM;
+#pragma unroll
+        for (int ch = 0; ch < NLM; ch++)
+            if (ch < nlm) {
+                if constexpr (sizeof(T) == 8) sum = sum + c[ch] * al[ch];
+                else sum = fmaf(c[ch], (float)al[ch], sum);
+            }
+    }
+    return sum;
+}
+
+// Node (K,J,I) of rank-box linear id (LIpos order).
+__device__ __forceinline__ void node_coords(const KParams &P, long id, int out[3])
+{
+    const int lx = P.box_n[0], ly = P.box_n[1];
+    out[0] = P.box0[0] + (int)(id % lx);
+    out[1] = P.box0[1] + (int)((id / lx) % ly);
+    out[2] = P.box0[2] + (int)(id / ((long)lx * ly));
+}
+
+template <typename T, int NLM>
+__global__ void __launch_bounds__(kWG) lanczos_kernel(KParams P)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    int *img_list = reinterpret_cast<int *>(smem);
+    double *A = reinterpret_cast<double *>(smem + P.lds_A);
+    double *alpha = reinterpret_cast<double *>(smem + P.lds_alpha);
+    double *red = reinterpret_cast<double *>(smem + P.lds_red);
+    short *local_of = reinterpret_cast<short *>(smem + P.lds_local);
+    __shared__ int s_count;
+
+    const int nx = P.nloc[0], ny = P.nloc[1];
+    const int nc = P.nc;
+    const float invx = 1.0f / (float)nx, invy = 1.0f / (float)ny;
+    // three vectors per workgroup: v, vprev (raw, with scales) and u = H v
+    T *buf0 = reinterpret_cast<T *>(P.scratch) + (size_t)blockIdx.x * 3 * nc;
+    T *buf1 = buf0 + nc;
+    T *Bu = buf1 + nc;
+    const T *diag_ext = reinterpret_cast<const T *>(P.diag_ext);
+    const T *chi = reinterpret_cast<const T *>(P.chi);
+
+    for (long r = blockIdx.x; r < P.node_count; r += gridDim.x) {
+        const long id = P.node_begin + r;
+        int node[3];
+        node_coords(P, id, node);
+        int cube_lo[3] = {node[0] - P.hw[0], node[1] - P.hw[1], node[2] - P.hw[2]};
+        // images whose box meets this cube, ascending (ordered compaction)
+        if (threadIdx.x == 0) s_count = 0;
+        __syncthreads();
+        for (int base = 0; base < P.n_images; base += kWG) {
+            const int g = base + threadIdx.x;
+            bool hit = false;
+            if (g < P.n_images) {
+                const int4 lo4 = P.img_lo[g], dm4 = P.img_dim[g];
+                hit = lo4.x <= cube_lo[0] + P.nloc[0] - 1 && lo4.x + dm4.x - 1 >= cube_lo[0] &&
+                      lo4.y <= cube_lo[1] + P.nloc[1] - 1 && lo4.y + dm4.y - 1 >= cube_lo[1] &&
+                      lo4.z <= cube_lo[2] + P.nloc[2] - 1 && lo4.z + dm4.z - 1 >= cube_lo[2];
+            }
+            const unsigned long long bal = __ballot(hit);
+            const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+            const int before = __popcll(bal & ((1ull << lane) - 1ull));
+            int *wcnt = reinterpret_cast<int *>(red);
+            if (lane == 0) wcnt[wave] = __popcll(bal);
+            __syncthreads();
+            int off = s_count;
+            for (int w = 0; w < wave; w++) off += wcnt[w];
+            if (hit) {
+                const int li = off + before;
+                if (li < P.img_cap) { img_list[li] = g; local_of[g] = (short)li; }
+            }
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                int t = s_count;
+                for (int w = 0; w < kWaves; w++) t += wcnt[w];
+                s_count = t;
+            }
+            __syncthreads();
+        }
+        const int nimg = s_count;
+        if (nimg > P.img_cap) {  // the host sized img_cap from the geometry; flag the node
+            if (threadIdx.x == 0) P.status[id] = 4;
+            __syncthreads();
+            continue;
+        }
+        // Lanczos start: v = e_center, vprev = 0
+        for (int p = threadIdx.x; p < nc; p += kWG) { buf0[p] = T(0); buf1[p] = T(0); }
+        __syncthreads();
+        if (threadIdx.x == 0) buf0[P.center] = T(1);
+        __syncthreads();
+        T *Bv = buf0, *Bvp = buf1;
+        T sv = T(1), svp = T(0);
+        double bprev = 0.0;
+        double *out_a = P.out_a + id * (P.nqp + 1), *out_b = P.out_b + id * (P.nqp + 1);
+        int status = 0;
+        for (int step = 0; step <= P.nqp; step++) {
+            gather_images<T, NLM>(P, img_list, nimg, cube_lo, Bv, sv, chi, A);
+            __syncthreads();
+            images_alpha<NLM>(P, img_list, nimg, A, alpha);
+            __syncthreads();
+            double pa = 0.0;
+            CSum ca{0.f, 0.f};
+            for (int p = threadIdx.x; p < nc; p += kWG) {
+                const int jk = fdiv(p, nx, invx);
+                const int li = p - jk * nx;
+                const int lk = fdiv(jk, ny, invy);
+                const int lj = jk - lk * ny;
+                const int64_t E = ((int64_t)(cube_lo[2] + lk - P.ext0[2]) * P.extn[1] + (cube_lo[1] + lj - P.ext0[1])) *
+                                      P.extn[0] + (cube_lo[0] + li - P.ext0[0]);
+                const T u = apply_point<T, NLM>(P, p, li, lj, lk, cube_lo, Bv, sv, diag_ext, chi, local_of, alpha, E);
+                Bu[p] = u;
+                const T vc = sv * Bv[p];
+                if constexpr (sizeof(T) == 8) pa += vc * u;
+                else ca = csum_add(ca, vc * u);
+            }
+            double a;
+            if constexpr (sizeof(T) == 8) a = block_sum(pa, red);
+            else a = block_csum(ca, red);
+            // w = (u - a v) - bprev vprev, written over vprev's storage
+            const T ta = T(-a), tb = T(-bprev);
+            double pb = 0.0;
+            CSum cb{0.f, 0.f};
+            for (int p = threadIdx.x; p < nc; p += kWG) {
+                const T vc = sv * Bv[p];
+                const T vpc = svp * Bvp[p];
+                const T u = Bu[p];
+                T w;
+                if constexpr (sizeof(T) == 8) {
+                    w = u + ta * vc;
+                    w = w + tb * vpc;
+                    pb += w * w;
+                } else {
+                    w = fmaf(ta, vc, u);
+                    w = fmaf(tb, vpc, w);
+                    cb = csum_add(cb, w * w);
+                }
+                Bvp[p] = w;
+            }
+            double bb;
+            if constexpr (sizeof(T) == 8) bb = block_sum(pb, red);
+            else bb = block_csum(cb, red);
+            const double b = sqrt(bb);
+            if (threadIdx.x == 0) { out_a[step] = a; out_b[step] = b; }
+            if (b == 0.0) { status = 7; break; }
+            // rotate: v <- w / b, vprev <- v
+            T *t = Bv; Bv = Bvp; Bvp = t;
+            svp = sv;
+            sv = T(1.0 / b);
+            bprev = b;
+        }
+        if (threadIdx.x == 0) P.status[id] = status;
+        __syncthreads();
+    }
+}
+
+// One H application y = H x on one node (device vectors x, y of nc entries).
+template <typename T, int NLM>
+__global__ void __launch_bounds__(kWG) apply_kernel(KParams P, long id, const T *x, T *y)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    int *img_list = reinterpret_cast<int *>(smem);
+    double *A = reinterpret_cast<double *>(smem + P.lds_A);
+    double *alpha = reinterpret_cast<double *>(smem + P.lds_alpha);
+    short *local_of = reinterpret_cast<short *>(smem + P.lds_local);
+    const int nx = P.nloc[0], ny = P.nloc[1];
+    const float invx = 1.0f / (float)nx, invy = 1.0f / (float)ny;
+    int node[3];
+    node_coords(P, id, node);
+    int cube_lo[3] = {node[0] - P.hw[0], node[1] - P.hw[1], node[2] - P.hw[2]};
+    if (threadIdx.x == 0) {
+        int n = 0;
+        for (int g = 0; g < P.n_images; g++) {
+            const int4 lo4 = P.img_lo[g], dm4 = P.img_dim[g];
+            bool hit = lo4.x <= cube_lo[0] + P.nloc[0] - 1 && lo4.x + dm4.x - 1 >= cube_lo[0] &&
+                       lo4.y <= cube_lo[1] + P.nloc[1] - 1 && lo4.y + dm4.y - 1 >= cube_lo[1] &&
+                       lo4.z <= cube_lo[2] + P.nloc[2] - 1 && lo4.z + dm4.z - 1 >= cube_lo[2];
+            if (hit && n < P.img_cap) { img_list[n] = g; local_of[g] = (short)n; }
+            n += hit;
+        }
+        reinterpret_cast<int *>(smem + P.lds_red)[0] = n;
+    }
+    __syncthreads();
+    const int nimg = reinterpret_cast<int *>(smem + P.lds_red)[0];
+    if (nimg > P.img_cap) { if (threadIdx.x == 0) P.status[0] = 4; return; }
+    __syncthreads();
+    gather_images<T, NLM>(P, img_list, nimg, cube_lo, x, T(1), reinterpret_cast<const T *>(P.chi), A);
+    __syncthreads();
+    images_alpha<NLM>(P, img_list, nimg, A, alpha);
+    __syncthreads();
+    for (int p = threadIdx.x; p < P.nc; p += blockDim.x) {
+        const int jk = fdiv(p, nx, invx);
+        const int li = p - jk * nx;
+        const int lk = fdiv(jk, ny, invy);
+        const int lj = jk - lk * ny;
+        const int64_t E = ((int64_t)(cube_lo[2] + lk - P.ext0[2]) * P.extn[1] + (cube_lo[1] + lj - P.ext0[1])) * P.extn[0] +
+                          (cube_lo[0] + li - P.ext0[0]);
+        y[p] = apply_point<T, NLM>(P, p, li, lj, lk, cube_lo, x, T(1), reinterpret_cast<const T *>(P.diag_ext),
+                                   reinterpret_cast<const T *>(P.chi), local_of, alpha, E);
+    }
+}
+
+// Implicit QL of one tridiagonal per thread (TridiagEigenVecSolve_NodesAndWeights,
+// lsdft_tools.cc:912-1026).  Only row 0 of the eigenvector matrix is carried:
+// the reference's rotations update each row of z independently, so row 0 -- the
+// one the weights w_i = z[0][i]^2 use -- is the same number sequence.
+__global__ void ql_kernel(const double *a, const double *bsub, int n, int stride, long count, double *lam, double *w,
+                          double *lmin, double *lmax, int *status, double *ws)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= count) return;
+    // d, e and row 0 of z live in a workspace laid out [i][thread] so that the
+    // threads of a wavefront touch consecutive addresses.
+    double *d = ws + t, *e = ws + (long)n * count + t, *z = ws + 2L * n * count + t;
+#define D(i) d[(long)(i) * count]
+#define EE(i) e[(long)(i) * count]
+#define Z(i) z[(long)(i) * count]
+    for (int i = 0; i < n; i++) {
+        D(i) = a[t * stride + i];
+        EE(i) = bsub[t * stride + i];
+        Z(i) = (i == 0) ? 1.0 : 0.0;
+    }
+    EE(n - 1) = 0.0;
+    int m, l, iter, i;
+    double s, r, p, g, f, dd, c, b;
+    for (l = 0; l <= n - 1; l++) {
+        iter = 0;
+        do {
+            for (m = l; m <= n - 2; m++) {
+                dd = fabs(D(m)) + fabs(D(m + 1));
+                if ((double)(fabs(EE(m)) + dd) == dd) break;
+            }
+            if (m != l) {
+                if (iter++ == 50) { status[t] = 5; return; }
+                g = (D(l + 1) - D(l)) / (2.0 * EE(l));
+                r = sqrt(g * g + 1.0);
+                g = D(m) - D(l) + EE(l) / (g + (g >= 0.0 ? fabs(r) : -fabs(r)));
+                s = c = 1.0;
+                p = 0.0;
+                for (i = m - 1; i >= l; i--) {
+                    f = s * EE(i);
+                    b = c * EE(i);
+                    EE(i + 1) = (r = sqrt(g * g + f * f));
+                    if (r == 0.0) {
+                        D(i + 1) -= p;
+                        EE(m) = 0.0;
+                        break;
+                    }
+                    s = f / r;
+                    c = g / r;
+                    g = D(i + 1) - p;
+                    r = (D(i) - g) * s + 2.0 * c * b;
+                    D(i + 1) = g + (p = s * r);
+                    g = c * r - b;
+                    f = Z(i + 1);
+                    Z(i + 1) = s * Z(i) + c * f;
+                    Z(i) = c * Z(i) - s * f;
+                }
+                if (r == 0.0 && i >= l) continue;
+                D(l) -= p;
+                EE(l) = g;
+                EE(m) = 0.0;
+            }
+        } while (m != l);
+    }
+    double emin = D(0), emax = D(0);
+    for (i = 0; i < n; i++) {
+        const double di = D(i), zi = Z(i);
+        lam[t * n + i] = di;
+        w[t * n + i] = zi * zi;
+        if (di > emax) emax = di;
+        else if (di < emin) emin = di;
+    }
+#undef D
+#undef EE
+#undef Z
+    lmin[t] = emin;
+    lmax[t] = emax;
+    status[t] = 0;
+}
+
+// ------------------------------------------------ density / Fermi / energy
+__device__ __forceinline__ double fermi_occ(double x)  // 1/(1+e^x)
+{
+    if (x > 0) { double t = exp(-x); return t / (1.0 + t); }
+    return 1.0 / (1.0 + exp(x));
+}
+__device__ __forceinline__ double fermi_ent(double x)  // f ln f + (1-f) ln(1-f)
+{
+    double ax = fabs(x), t = exp(-ax);
+    return -log1p(t) - ax * t / (1.0 + t);
+}
+
+constexpr int kRedThreads = 256;
+
+// partial[block] = sum over the block's chunk of w * f(beta (lam - lf))
+__global__ void __launch_bounds__(kRedThreads) occupation_partial(const double *lam, const double *w, long nq, double beta,
+                                                                  double lf, double *partial)
+{
+    __shared__ double red[kRedThreads / 64];
+    const long chunk = (nq + gridDim.x - 1) / gridDim.x;
+    const long q0 = blockIdx.x * chunk, q1 = min(nq, q0 + chunk);
+    double s = 0.0;
+    for (long q = q0 + threadIdx.x; q < q1; q += blockDim.x) s += w[q] * fermi_occ(beta * (lam[q] - lf));
+    s = wave_sum(s);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int k = 0; k < kRedThreads / 64; k++) t += red[k];
+        partial[blockIdx.x] = t;
+    }
+}
+
+// sum of partials in index order -> out[0]
+__global__ void ordered_sum(const double *partial, int n, double scale, double *out)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        double s = 0.0;
+        for (int k = 0; k < n; k++) s += partial[k];
+        out[0] = scale * s;
+    }
+}
+
+// per node: rho = (2/dV) sum_i w f; partials of sum w lam f and sum w S(f)
+__global__ void __launch_bounds__(kRedThreads) density_energy(const double *lam, const double *w, long n_nodes, int nqp,
+                                                              double beta, double lf, double two_over_dv, double *rho,
+                                                              double *part_e, double *part_s)
+{
+    __shared__ double re[kRedThreads / 64], rs[kRedThreads / 64];
+    const long chunk = (n_nodes + gridDim.x - 1) / gridDim.x;
+    const long p0 = blockIdx.x * chunk, p1 = min(n_nodes, p0 + chunk);
+    double se = 0.0, ss = 0.0;
+    for (long p = p0 + threadIdx.x; p < p1; p += blockDim.x) {
+        double sr = 0.0;
+        for (int i = 0; i < nqp; i++) {
+            const long q = p * nqp + i;
+            const double x = beta * (lam[q] - lf);
+            const double f = fermi_occ(x);
+            sr += w[q] * f;
+            se += w[q] * lam[q] * f;
+            ss += w[q] * fermi_ent(x);
+        }
+        if (rho) rho[p] = two_over_dv * sr;
+    }
+    se = wave_sum(se);
+    ss = wave_sum(ss);
+    if ((threadIdx.x & 63) == 0) { re[threadIdx.x >> 6] = se; rs[threadIdx.x >> 6] = ss; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double a = 0.0, b = 0.0;
+        for (int k = 0; k < kRedThreads / 64; k++) { a += re[k]; b += rs[k]; }
+        part_e[blockIdx.x] = a;
+        part_s[blockIdx.x] = b;
+    }
+}
+
+// min/max of lambda over all nodes (bracket of the Fermi level)
+__global__ void lam_range(const double *lmin, const double *lmax, long n, double *out)
+{
+    __shared__ double smin[kRedThreads], smax[kRedThreads];
+    double a = 1e300, b = -1e300;
+    for (long p = threadIdx.x; p < n; p += blockDim.x) { a = fmin(a, lmin[p]); b = fmax(b, lmax[p]); }
+    smin[threadIdx.x] = a;
+    smax[threadIdx.x] = b;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int k = 1; k < (int)blockDim.x; k++) { a = fmin(a, smin[k]); b = fmax(b, smax[k]); }
+        out[0] = a;
+        out[1] = b;
+    }
+}
+
+// ------------------------------------------------------------ launchers
+template <typename T, int NLM>
+static hipError_t launch_lanczos_nlm(const KParams &P, int grid, size_t lds, hipStream_t st)
+{
+    hipLaunchKernelGGL((lanczos_kernel<T, NLM>), dim3(grid), dim3(kWG), lds, st, P);
+    return hipGetLastError();
+}
+
+hipError_t launch_lanczos(const KParams &P, int precision, int grid, size_t lds, hipStream_t st)
+{
+    if (P.nlm_max <= 6) {
+        if (precision == 0) return launch_lanczos_nlm<double, 6>(P, grid, lds, st);
+        return launch_lanczos_nlm<float, 6>(P, grid, lds, st);
+    }
+    if (P.nlm_max <= kMaxChannels) {
+        if (precision == 0) return launch_lanczos_nlm<double, kMaxChannels>(P, grid, lds, st);
+        return launch_lanczos_nlm<float, kMaxChannels>(P, grid, lds, st);
+    }
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_apply(const KParams &P, long id, const double *x, double *y, size_t lds, hipStream_t st)
+{
+    if (P.nlm_max <= 6) hipLaunchKernelGGL((apply_kernel<double, 6>), dim3(1), dim3(kWG), lds, st, P, id, x, y);
+    else if (P.nlm_max <= kMaxChannels)
+        hipLaunchKernelGGL((apply_kernel<double, kMaxChannels>), dim3(1), dim3(kWG), lds, st, P, id, x, y);
+    else return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+hipError_t launch_ql(const double *a, const double *b, int n, int stride, long count, double *lam, double *w,
+                     double *lmin, double *lmax, int *status, double *ws, hipStream_t st)
+{
+    if (n < 1 || n > kMaxNqp || count <= 0) return hipErrorInvalidValue;
+    const int threads = 64;
+    const long blocks = (count + threads - 1) / threads;
+    hipLaunchKernelGGL(ql_kernel, dim3((unsigned)blocks), dim3(threads), 0, st, a, b, n, stride, count, lam, w, lmin,
+                       lmax, status, ws);
+    return hipGetLastError();
+}
+
+hipError_t launch_occupation(const double *lam, const double *w, long nq, double beta, double lf, double *partial,
+                             int nblocks, double *out, hipStream_t st)
+{
+    hipLaunchKernelGGL(occupation_partial, dim3(nblocks), dim3(kRedThreads), 0, st, lam, w, nq, beta, lf, partial);
+    hipLaunchKernelGGL(ordered_sum, dim3(1), dim3(64), 0, st, partial, nblocks, 2.0, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_density_energy(const double *lam, const double *w, long n_nodes, int nqp, double beta, double lf,
+                                 double two_over_dv, double *rho, double *part_e, double *part_s, int nblocks,
+                                 double *out2, hipStream_t st)
+{
+    hipLaunchKernelGGL(density_energy, dim3(nblocks), dim3(kRedThreads), 0, st, lam, w, n_nodes, nqp, beta, lf,
+                       two_over_dv, rho, part_e, part_s);
+    hipLaunchKernelGGL(ordered_sum, dim3(1), dim3(64), 0, st, part_e, nblocks, 2.0, out2);
+    hipLaunchKernelGGL(ordered_sum, dim3(1), dim3(64), 0, st, part_s, nblocks, 2.0 / beta, out2 + 1);
+    return hipGetLastError();
+}
+
+hipError_t launch_lam_range(const double *lmin, const double *lmax, long n, double *out, hipStream_t st)
+{
+    hipLaunchKernelGGL(lam_range, dim3(1), dim3(kRedThreads), 0, st, lmin, lmax, n, out);
+    return hipGetLastError();
+}
+
+}  // namespace sqamd

@@ -1,0 +1,614 @@
+// sq_setup.cpp -- PETSc-free host setup of the SQ hot path.
+//
+// Builds, from an SQMiniApp input deck, everything the device path consumes:
+// grid and FD weights, pseudopotential splines and KB coefficients, the
+// pseudocharge cutoff that fixes the periodic image range, the projector
+// images with their dense chi boxes, and, per point of the extended rank box,
+// the ascending list of images whose box holds it.  The arithmetic of every
+// quantity follows the reference function cited at it, so chi is bit-identical
+// to the reference's (the cube, image and spline steps have to be alike).
+#include "sq_system.h"
+
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <sstream>
+#include <thread>
+
+namespace sqamd {
+
+namespace {
+
+Status fail(int code, const std::string &msg) { return Status{code, msg}; }
+
+// ------------------------------------------------------------------ splines
+// Not-a-knot slopes of a cubic spline (Wolberg's formulation, the one of
+// getYD_gen/tridiag_gen, lsdft_tools.cc:75-151): a tridiagonal system solved
+// by forward elimination and back substitution, in the same operation order.
+bool notaknot_slopes(const double *X, const double *Y, int n, std::vector<double> &YD)
+{
+    YD.assign(n, 0.0);
+    std::vector<double> A(n, 0.0), B(n, 0.0), Cc(n, 0.0), F(n, 0.0);
+    double h0 = X[1] - X[0], h1 = X[2] - X[1];
+    double r0 = (Y[1] - Y[0]) / h0, r1 = (Y[2] - Y[1]) / h1;
+    B[0] = h1 * (h0 + h1);
+    Cc[0] = (h0 + h1) * (h0 + h1);
+    YD[0] = r0 * (3 * h0 * h1 + 2 * h1 * h1) + r1 * h0 * h0;
+    int i = 1;
+    for (; i < n - 1; i++) {
+        h0 = X[i] - X[i - 1];
+        h1 = X[i + 1] - X[i];
+        r0 = (Y[i] - Y[i - 1]) / h0;
+        r1 = (Y[i + 1] - Y[i]) / h1;
+        A[i] = h1;
+        B[i] = 2 * (h0 + h1);
+        Cc[i] = h0;
+        YD[i] = 3 * (r0 * h1 + r1 * h0);
+    }
+    A[i] = (h0 + h1) * (h0 + h1);
+    B[i] = h0 * (h0 + h1);
+    YD[i] = r0 * h1 * h1 + r1 * (3 * h0 * h1 + 2 * h0 * h0);
+    double b = B[0];
+    YD[0] = YD[0] / b;
+    for (int k = 1; k < n; k++) {
+        F[k] = Cc[k - 1] / b;
+        b = B[k] - A[k] * F[k];
+        if (b == 0) return false;
+        YD[k] = (YD[k] - YD[k - 1] * A[k]) / b;
+    }
+    for (int k = n - 2; k >= 0; k--) YD[k] -= (YD[k + 1] * F[k + 1]);
+    return true;
+}
+
+}  // namespace
+
+// Cubic Hermite evaluation on the interval holding r (SplineEvaluate,
+// lsdft_tools.cc:15-68).  The interval is the one the reference's linear scan
+// stops at: the first knot >= r, stepped back unless r sits on it.
+bool RadialSpline::eval(double r, double *out) const
+{
+    if (!(r >= x[0] && r <= x[n - 1])) return false;
+    int j = (int)(std::lower_bound(x, x + n, r) - x);
+    if (r < x[j]) j--;
+    double p1 = x[j];
+    double dx = 1.0 / (x[j + 1] - x[j]);
+    double dy = (y[j + 1] - y[j]) * dx;
+    double A0 = y[j], A1 = yd[j];
+    double A2 = dx * (3.0 * dy - 2.0 * yd[j] - yd[j + 1]);
+    double A3 = dx * dx * (-2.0 * dy + yd[j] + yd[j + 1]);
+    double t = r - p1;
+    *out = ((A3 * t + A2) * t + A1) * t + A0;
+    return true;
+}
+
+// (n!)^2/((n-k)!(n+k)!)  (fract, lsdft_tools.cc:519-531)
+double fd_fract(int n, int k)
+{
+    double num = 1, den = 1;
+    for (int i = n - k + 1; i <= n; i++) num *= i;
+    for (int i = n + 1; i <= n + k; i++) den *= i;
+    return num / den;
+}
+
+// Real spherical harmonics for l <= 3 with the reference's coefficient table
+// (RealSphericalHarmonic, lsdft_tools.cc:156-240): the same 15-digit
+// constants and the same expressions, so projector values match bit for bit.
+double real_spherical_harmonic(double x, double y, double z, int l, int m, bool *bad)
+{
+    static const double c00 = 0.282094791773878;
+    static const double c1 = 0.488602511902920;
+    static const double c2a = 1.092548430592079, c20 = 0.315391565252520, c22 = 0.546274215296040;
+    static const double c33 = 0.590043589926644, c32n = 2.890611442640554, c31 = 0.457045799464466;
+    static const double c30 = 0.373176332590115, c32p = 1.445305721320277;
+    *bad = false;
+    double r = std::sqrt(x * x + y * y + z * z);
+    if (l == 0) return c00;
+    if (l > 3 || m < -l || m > l) { *bad = true; return 0.0; }
+    if (!(r >= 1e-10)) return 0.0;
+    switch (l * 10 + (m + 3)) {
+    case 12: return c1 * (y / r);
+    case 13: return c1 * (z / r);
+    case 14: return c1 * (x / r);
+    case 21: return c2a * (x * y) / (r * r);
+    case 22: return c2a * (y * z) / (r * r);
+    case 23: return c20 * (-x * x - y * y + 2.0 * z * z) / (r * r);
+    case 24: return c2a * (z * x) / (r * r);
+    case 25: return c22 * (x * x - y * y) / (r * r);
+    case 30: return c33 * (3 * x * x - y * y) * y / (r * r * r);
+    case 31: return c32n * (x * y * z) / (r * r * r);
+    case 32: return c31 * y * (4 * z * z - x * x - y * y) / (r * r * r);
+    case 33: return c30 * z * (2 * z * z - 3 * x * x - 3 * y * y) / (r * r * r);
+    case 34: return c31 * x * (4 * z * z - x * x - y * y) / (r * r * r);
+    case 35: return c32p * z * (x * x - y * y) / (r * r * r);
+    case 36: return c33 * x * (x * x - 3 * y * y) / (r * r * r);
+    }
+    *bad = true;
+    return 0.0;
+}
+
+namespace {
+
+// ------------------------------------------------------------------- decks
+bool read_file(const std::string &path, std::string &out)
+{
+    std::ifstream f(path, std::ios::binary);
+    if (!f) return false;
+    std::ostringstream ss;
+    ss << f.rdbuf();
+    out = ss.str();
+    return true;
+}
+
+// .inpt keyword scan (Read_Parameters, lsdft_readfiles.cc:148-331, with the
+// defaults of 62-102); derived h, n_loc and FD weights (356-476).
+Status parse_inpt(const std::string &path, System &S)
+{
+    std::string text;
+    if (!read_file(path, text)) return fail(1, "cannot open " + path);
+    std::istringstream in(text);
+    std::string tok;
+    auto need = [&](auto &v) -> bool { return (bool)(in >> v); };
+    while (in >> tok) {
+        bool ok = true;
+        if (tok == "FD_ORDER:") { ok = need(S.order); S.order /= 2; }
+        else if (tok == "BOUNDARY_CONDITION:") ok = need(S.bc);
+        else if (tok == "CELL:") ok = need(S.range[0]) && need(S.range[1]) && need(S.range[2]);
+        else if (tok == "FD_GRID:") ok = need(S.npts[0]) && need(S.npts[1]) && need(S.npts[2]);
+        else if (tok == "BETA:") ok = need(S.beta);
+        else if (tok == "RADIUS_TRUNC:") ok = need(S.rcut);
+        else if (tok == "LANCZOS_DEGREE:") ok = need(S.nqp);
+        else if (tok == "TOL_PSEUDOCHARGE:") ok = need(S.tol_psc);
+        else if (tok == "NTYPE:") ok = need(S.ntype) && S.ntype >= 1 && S.ntype <= kMaxType;
+        else if (tok == "PSEUDOPOTENTIAL_LOCAL:") {
+            ok = S.ntype > 0;
+            for (int i = 0; ok && i < S.ntype; i++) ok = need(S.localPsd[i]);
+        } else if (tok == "PSEUDOPOTENTIAL_FILE:") {
+            ok = S.ntype > 0;
+            for (int i = 0; ok && i < S.ntype; i++) ok = need(S.psdFile[i]);
+        }
+        if (!ok) return fail(2, "malformed value after " + tok + " in " + path);
+    }
+    if (S.ntype <= 0) return fail(2, "NTYPE missing in " + path);
+    if (S.order < 1 || S.order > kMaxOrder) return fail(2, "FD_ORDER out of range in " + path);
+    if (S.bc != 1) return fail(4, "only BOUNDARY_CONDITION 1 is on the path");
+    for (int d = 0; d < 3; d++)
+        if (S.npts[d] < 2 || !(S.range[d] > 0)) return fail(2, "bad CELL/FD_GRID in " + path);
+    if (S.nqp < 1) return fail(2, "LANCZOS_DEGREE must be positive");
+    for (int d = 0; d < 3; d++) {
+        S.h[d] = 2.0 * S.range[d] / (S.npts[d] - 1);
+        S.hw[d] = (int)std::ceil(S.rcut / S.h[d]);
+        S.nloc[d] = 2 * S.hw[d] + 1;
+    }
+    S.dV = S.h[0] * S.h[1] * S.h[2];
+    double c0 = 0;
+    for (int a = 1; a <= S.order; a++) c0 += ((double)1.0 / (a * a));
+    for (int d = 0; d < 3; d++) {
+        S.coeff[d][0] = c0;
+        S.coeff[d][0] *= ((double)1.0 / (S.h[d] * S.h[d]));
+        for (int p = 1; p <= S.order; p++)
+            S.coeff[d][p] = (double)(-1 * std::pow(-1, p + 1) * fd_fract(S.order, p) / (p * p * S.h[d] * S.h[d]));
+    }
+    S.cdiag = S.coeff[0][0] + S.coeff[1][0] + S.coeff[2][0];
+    return {};
+}
+
+// .ion blocks "<type> <valence> <count>" + positions (Read_Ion, lsdft_readfiles.cc:677-789)
+Status parse_ion(const std::string &path, System &S)
+{
+    std::string text;
+    if (!read_file(path, text)) return fail(1, "cannot open " + path);
+    std::istringstream in(text);
+    if (!(in >> S.nAtoms) || S.nAtoms < 0) return fail(2, "atom count missing in " + path);
+    int at = 0;
+    S.noetot = 0;
+    std::string name;
+    while (at < kMaxType && (in >> name)) {
+        if (!(in >> S.noe[at] >> S.noa[at])) break;
+        S.atomType[at] = name;
+        for (int k = 0; k < S.noa[at]; k++) {
+            double x, y, z;
+            if (!(in >> x >> y >> z)) return fail(2, "truncated positions in " + path);
+            S.atompos.push_back(x); S.atompos.push_back(y); S.atompos.push_back(z);
+            S.atomtype.push_back(at);
+            S.noetot += S.noe[at];
+        }
+        at++;
+    }
+    if ((int)S.atomtype.size() != S.nAtoms) return fail(2, "atom count mismatch in " + path);
+    if (at != S.ntype) return fail(2, "number of atom blocks differs from NTYPE in " + path);
+    return {};
+}
+
+// Troullier-Martins .pot (Read_Pseudopotential, lsdft_readfiles.cc:858-1101):
+// radial grid with r=0 prepended; V_l = 0.5 (r V_l)/r (Ry -> Ha), U_l = u_l/r,
+// rho = q/(4 pi r^2); at r=0, V and U repeat their first value and rho is 0.
+Status parse_pot(const std::string &path, const std::string &element, int lloc, Pseudopotential &P)
+{
+    std::string text;
+    if (!read_file(path, text)) return fail(1, "cannot open " + path);
+    // Token stream in which the block header lines become marker tokens.
+    enum Kind { NUM, RADIAL, PSP, CORE, VALENCE, WAVE, OTHER };
+    struct Tok { Kind k; std::string s; };
+    std::vector<Tok> toks;
+    std::istringstream lines(text);
+    std::string line;
+    bool first = true;
+    std::string head;
+    while (std::getline(lines, line)) {
+        std::string t = line;
+        t.erase(0, t.find_first_not_of(" \t\r"));
+        if (first) { head = t.substr(0, t.find_first_of(" \t\r")); first = false; continue; }
+        if (t.rfind("Radial grid follows", 0) == 0) { toks.push_back({RADIAL, ""}); continue; }
+        if (t.rfind("Pseudopotential follows", 0) == 0) { toks.push_back({PSP, ""}); continue; }
+        if (t.rfind("Core charge follows", 0) == 0) { toks.push_back({CORE, ""}); continue; }
+        if (t.rfind("Valence charge follows", 0) == 0) { toks.push_back({VALENCE, ""}); continue; }
+        if (t.rfind("Pseudo-wave-function follows", 0) == 0) { toks.push_back({WAVE, ""}); continue; }
+        std::istringstream ls(line);
+        std::string w;
+        while (ls >> w) toks.push_back({NUM, w});
+    }
+    if (head != element) return fail(2, path + " is not a pseudopotential of " + element);
+    size_t i = 0;
+    while (i < toks.size() && toks[i].k != RADIAL) i++;
+    if (i == toks.size()) return fail(2, "no radial grid in " + path);
+    i++;
+    std::vector<double> rv;
+    while (i < toks.size() && toks[i].k == NUM) rv.push_back(std::strtod(toks[i++].s.c_str(), nullptr));
+    int count = (int)rv.size();
+    if (count < 3) return fail(2, "radial grid too short in " + path);
+    P.element = element;
+    P.size = count + 1;
+    P.r.assign(P.size, 0.0);
+    for (int k = 0; k < count; k++) P.r[k + 1] = rv[k];
+    P.uu.assign(P.size, 0.0);
+    P.vloc.assign(P.size, 0.0);
+    for (int l = 0; l < 4; l++) { P.V[l].assign(P.size, 0.0); P.U[l].assign(P.size, 0.0); P.rc[l] = 0.0; }
+    auto take = [&](double &v) -> bool {
+        if (i >= toks.size() || toks[i].k != NUM) return false;
+        v = std::strtod(toks[i++].s.c_str(), nullptr);
+        return true;
+    };
+    bool have_valence = false;
+    P.lmax = -1;
+    while (i < toks.size()) {
+        Kind k = toks[i++].k;
+        if (k == PSP || k == WAVE) {
+            double lv, zel = 0, rc = 0;
+            if (!take(lv)) return fail(2, "missing l in " + path);
+            int l = (int)lv;
+            if (l < 0 || l > 3) return fail(2, "l out of range in " + path);
+            if (k == WAVE && !(take(zel) && take(rc))) return fail(2, "missing zelect/rc in " + path);
+            for (int q = 1; q <= count; q++) {
+                double v;
+                if (!take(v)) return fail(2, "truncated block in " + path);
+                if (k == PSP) {
+                    v = 0.5 * v / (P.r[q]);
+                    P.V[l][q] = v;
+                    if (lloc == l) P.vloc[q] = v;
+                } else {
+                    v = v / P.r[q];
+                    P.U[l][q] = v;
+                }
+            }
+            if (k == PSP) {
+                P.V[l][0] = P.V[l][1];
+                if (lloc == l) P.vloc[0] = P.V[l][1];
+                P.lmax = std::max(P.lmax, l);
+            } else {
+                P.rc[l] = rc;
+                P.U[l][0] = P.U[l][1];
+            }
+        } else if (k == VALENCE) {
+            for (int q = 1; q <= count; q++) {
+                double v;
+                if (!take(v)) return fail(2, "truncated valence charge in " + path);
+                P.uu[q] = v / (4 * M_PI * P.r[q] * P.r[q]);
+            }
+            P.uu[0] = 0.0;
+            have_valence = true;
+        } else if (k == CORE) {
+            while (i < toks.size() && toks[i].k == NUM) i++;
+        }
+    }
+    if (P.lmax < 0 || lloc > P.lmax || !have_valence) return fail(2, "incomplete pseudopotential " + path);
+    P.lloc = lloc;
+    return {};
+}
+
+// Spline fits (Calculate_SplineFit, lsdft_initObjs.cc:576-622) and Gamma_l
+// (Calculate_NonlocalCoefficients, lsdft_initObjs.cc:822-869): a rectangle
+// rule, dr = 1e-6, of U dV U r^2 to rc_l + max h, half of the last sample
+// removed, divided by dV, inverted.
+Status fit_pseudopotential(const System &S, Pseudopotential &P)
+{
+    auto fit = [&](RadialSpline &s, const std::vector<double> &y) {
+        s.x = P.r.data();
+        s.n = P.size;
+        s.y = y;
+        return notaknot_slopes(P.r.data(), y.data(), P.size, s.yd);
+    };
+    if (!fit(P.s_vloc, P.vloc)) return fail(3, "singular spline system (Vloc)");
+    for (int l = 0; l <= P.lmax; l++) {
+        std::vector<double> udv(P.size);
+        for (int q = 0; q < P.size; q++) udv[q] = P.U[l][q] * (P.V[l][q] - P.vloc[q]);
+        if (!fit(P.s_udv[l], udv) || !fit(P.s_u[l], P.U[l])) return fail(3, "singular spline system");
+    }
+    const double dr = 1e-6;
+    double a = S.h[0], b = S.h[1], c = S.h[2];
+    double delta = (a > b && a > c ? a : b > c ? b : c);
+    for (int l = 0; l <= P.lmax; l++) {
+        P.gamma[l] = 0.0;
+        if (l == P.lloc) continue;
+        double rt = dr, D = 0.0, f1 = 0.0, f2 = 0.0;
+        while (rt <= (P.rc[l] + delta)) {
+            if (!P.s_udv[l].eval(rt, &f1) || !P.s_u[l].eval(rt, &f2)) return fail(3, "Gamma quadrature off the radial grid");
+            D += f1 * f2 * rt * rt * dr;
+            rt = rt + dr;
+        }
+        D = (D - 0.5 * f1 * f2 * (rt - dr) * (rt - dr) * dr) / (S.dV);
+        P.gamma[l] = 1.0 / D;
+    }
+    P.ch_l.clear();
+    P.ch_m.clear();
+    for (int l = 0; l <= P.lmax; l++) {
+        if (l == P.lloc) continue;
+        for (int m = -l; m <= l; m++) { P.ch_l.push_back(l); P.ch_m.push_back(m); }
+    }
+    return {};
+}
+
+// Pseudocharge cutoff (Calculate_PseudochargeCutoff, lsdft_initObjs.cc:628-817).
+// Only its result is used here: ceil(cutoff/R) bounds the periodic images.
+Status pseudocharge_cutoff(System &S, int at)
+{
+    const Pseudopotential &P = S.psd[at];
+    const int o = S.order;
+    int Np[3];
+    for (int d = 0; d < 3; d++) Np[d] = (int)std::ceil(15.0 / S.h[d]);
+    const int Ex = Np[0] + 2 * o, Ey = Np[1] + 2 * o, Ez = Np[2] + 2 * o;
+    std::vector<double> vps((size_t)Ex * Ey * Ez), bj((size_t)Np[0] * Np[1] * Np[2]);
+    auto VP = [&](int k, int j, int i) -> double & { return vps[((size_t)k * Ey + j) * Ex + i]; };
+    for (int k = 0; k < Ez; k++)
+        for (int j = 0; j < Ey; j++)
+            for (int i = 0; i < Ex; i++) {
+                double x = (i - o) * S.h[0], y = (j - o) * S.h[1], z = (k - o) * S.h[2];
+                double r = std::sqrt(x * x + y * y + z * z);
+                if (!P.s_vloc.eval(r, &VP(k, j, i))) return fail(3, "pseudocharge grid beyond the radial table");
+            }
+    double sc[3][kMaxOrder + 1];
+    for (int d = 0; d < 3; d++)
+        for (int p = 0; p <= o; p++) sc[d][p] = S.coeff[d][p] / (2.0 * M_PI);
+    for (int k = o; k < Np[2] + o; k++)
+        for (int j = o; j < Np[1] + o; j++)
+            for (int i = o; i < Np[0] + o; i++) {
+                double v = VP(k, j, i) * (sc[0][0] + sc[1][0] + sc[2][0]);
+                for (int a = 1; a <= o; a++)
+                    v += (VP(k, j, i - a) + VP(k, j, i + a)) * sc[0][a] + (VP(k, j - a, i) + VP(k, j + a, i)) * sc[1][a] +
+                         (VP(k - a, j, i) + VP(k + a, j, i)) * sc[2][a];
+                bj[((size_t)(k - o) * Np[1] + (j - o)) * Np[0] + (i - o)] = v;
+            }
+    double Rc = P.rc[P.lloc], err;
+    int Nc[3];
+    do {
+        for (int d = 0; d < 3; d++) Nc[d] = (int)std::ceil(Rc / S.h[d]);
+        double B = 0.0;
+        for (int k = 0; k < Nc[2]; k++)
+            for (int j = 0; j < Nc[1]; j++)
+                for (int i = 0; i < Nc[0]; i++) {
+                    double v = bj[((size_t)k * Np[1] + j) * Np[0] + i];
+                    if (k == 0) v = 0.5 * v;
+                    if (k == Nc[2] - 1) v = 0.5 * v;
+                    if (j == 0) v = 0.5 * v;
+                    if (j == Nc[1] - 1) v = 0.5 * v;
+                    if (i == 0) v = 0.5 * v;
+                    if (i == Nc[0] - 1) v = 0.5 * v;
+                    B += v;
+                }
+        B = B * 8.0 * S.dV;
+        err = std::fabs(B + S.noe[at]) / std::fabs((double)S.noe[at]);
+        Rc = Rc + 1.0;
+    } while (Rc <= 15.0 && err > S.tol_psc);
+    for (int d = 0; d < 3; d++) S.cutoff[at][d] = Nc[d] * S.h[d];
+    return {};
+}
+
+// Clip [s,e] to the window [w0, w0+n-1] the way Check_Overlap
+// (lsdft_initObjs.cc:2612-2663) does; false when an end stays unset.
+bool clip_axis(int w0, int n, int s, int e, int *cs, int *ce)
+{
+    const int unset = -1000;
+    int a = unset, b = unset;
+    if (s >= w0 && s <= w0 + n - 1) a = s;
+    else if (w0 >= s && w0 <= e) a = w0;
+    if (e >= w0 && e <= w0 + n - 1) b = e;
+    else if (w0 + n - 1 >= s && w0 + n - 1 <= e) b = w0 + n - 1;
+    *cs = a;
+    *ce = b;
+    return a != unset && b != unset;
+}
+
+// Nearest grid index of a coordinate, rounded in single precision as the
+// reference's roundf((x0 + R)/h) does (lsdft_initObjs.cc:1800-1802).
+int nearest_index(double v) { return (int)roundf((float)v); }
+
+// Images of every supercell atom inside the periodic range that overlap the
+// extended rank box, in the reference's order (atoms, then P, Q, R;
+// GetInfluencingAtoms_Nonlocal, lsdft_initObjs.cc:1671-1946).
+void collect_images(System &S)
+{
+    S.images.clear();
+    const int nsc = (int)S.sctype.size();
+    for (int at = 0; at < S.ntype; at++) {
+        const Pseudopotential &P = S.psd[at];
+        if (P.lmax == 0) continue;
+        double m1 = P.rc[0] > P.rc[1] ? P.rc[0] : P.rc[1];
+        double m2 = P.rc[2] > P.rc[3] ? P.rc[2] : P.rc[3];
+        double cut = m1 > m2 ? m1 : m2;
+        int off[3], Im[3];
+        for (int d = 0; d < 3; d++) {
+            off[d] = (int)std::ceil(cut / S.h[d]);
+            Im[d] = (int)std::ceil(S.cutoff[at][d] / S.range[d]);
+        }
+        for (int p = 0; p < nsc; p++) {
+            if (S.sctype[p] != at) continue;
+            const double X0 = S.scpos[3 * p], Y0 = S.scpos[3 * p + 1], Z0 = S.scpos[3 * p + 2];
+            for (int PP = -Im[0]; PP <= Im[0]; PP++)
+                for (int QQ = -Im[1]; QQ <= Im[1]; QQ++)
+                    for (int RR = -Im[2]; RR <= Im[2]; RR++) {
+                        double x0 = X0 + PP * 2.0 * S.range[0];
+                        double y0 = Y0 + QQ * 2.0 * S.range[1];
+                        double z0 = Z0 + RR * 2.0 * S.range[2];
+                        int c[3] = {nearest_index((x0 + S.range[0] + 0 * S.rcut) / S.h[0]),
+                                    nearest_index((y0 + S.range[1] + 0 * S.rcut) / S.h[1]),
+                                    nearest_index((z0 + S.range[2] + 0 * S.rcut) / S.h[2])};
+                        Image im;
+                        bool ok = true;
+                        for (int d = 0; d < 3 && ok; d++)
+                            ok = clip_axis(S.ext0[d], S.extn[d], c[d] - off[d], c[d] + off[d], &im.lo[d], &im.hi[d]);
+                        if (!ok) continue;
+                        im.type = at;
+                        im.atom = p;
+                        im.X0 = x0; im.Y0 = y0; im.Z0 = z0;
+                        im.nlm = P.nlm();
+                        S.images.push_back(im);
+                    }
+        }
+    }
+}
+
+// chi_lm(x) = S_l(|x - R|) Y_lm(x - R) on each image box, stored with the box
+// point outer and channel inner (CalculateNonlocalProjectors,
+// lsdft_nl.cc:141-208, flattened as NonlocalProjectors, lsdft_nl.cc:81-123).
+Status fill_projectors(System &S)
+{
+    int64_t total = 0;
+    S.max_nlm = 0;
+    for (Image &im : S.images) {
+        im.chi_off = total;
+        total += im.points() * im.nlm;
+        S.max_nlm = std::max(S.max_nlm, im.nlm);
+    }
+    S.chi.assign((size_t)total, 0.0);
+    std::atomic<int> next{0}, bad{0};
+    auto worker = [&]() {
+        for (;;) {
+            int c = next.fetch_add(1);
+            if (c >= (int)S.images.size() || bad.load()) return;
+            const Image &im = S.images[c];
+            const Pseudopotential &P = S.psd[im.type];
+            double *out = S.chi.data() + im.chi_off;
+            for (int k = im.lo[2]; k <= im.hi[2]; k++) {
+                double z = S.h[2] * k - S.range[2] - im.Z0, zz = z * z;
+                for (int j = im.lo[1]; j <= im.hi[1]; j++) {
+                    double y = S.h[1] * j - S.range[1] - im.Y0, yy = y * y;
+                    for (int i = im.lo[0]; i <= im.hi[0]; i++) {
+                        double x = S.h[0] * i - S.range[0] - im.X0;
+                        double r = std::sqrt(x * x + yy + zz);
+                        double radial[4] = {0, 0, 0, 0};
+                        for (int l = 0; l <= P.lmax; l++) {
+                            if (l == P.lloc) continue;
+                            if (!P.s_udv[l].eval(r, &radial[l])) { bad = 3; return; }
+                        }
+                        for (int ch = 0; ch < im.nlm; ch++) {
+                            bool b = false;
+                            double sh = real_spherical_harmonic(x, y, z, P.ch_l[ch], P.ch_m[ch], &b);
+                            if (b) { bad = 4; return; }
+                            *out++ = radial[P.ch_l[ch]] * sh;
+                        }
+                    }
+                }
+            }
+        }
+    };
+    unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    std::vector<std::thread> pool;
+    for (unsigned t = 0; t < nt; t++) pool.emplace_back(worker);
+    for (auto &t : pool) t.join();
+    if (bad) return fail(bad.load(), "projector evaluation failed (radius off the table or bad l)");
+    return {};
+}
+
+// For every point of the extended box, the ascending list of images whose box
+// holds it: the order in which the reference adds chi*alpha into a point
+// (lsdft_matvec.cc:110-142).
+void build_coverage(System &S)
+{
+    const int64_t np = S.ext_points();
+    std::vector<int32_t> cnt(np + 1, 0);
+    auto eidx = [&](int i, int j, int k) {
+        return ((int64_t)(k - S.ext0[2]) * S.extn[1] + (j - S.ext0[1])) * S.extn[0] + (i - S.ext0[0]);
+    };
+    for (const Image &im : S.images)
+        for (int k = im.lo[2]; k <= im.hi[2]; k++)
+            for (int j = im.lo[1]; j <= im.hi[1]; j++)
+                for (int i = im.lo[0]; i <= im.hi[0]; i++) cnt[eidx(i, j, k) + 1]++;
+    S.max_cov = 0;
+    for (int64_t p = 0; p < np; p++) S.max_cov = std::max(S.max_cov, cnt[p + 1]);
+    for (int64_t p = 0; p < np; p++) cnt[p + 1] += cnt[p];
+    S.cov_off = cnt;
+    S.cov_img.assign(cnt[np], 0);
+    std::vector<int32_t> fillp(cnt.begin(), cnt.end() - 1);
+    for (int c = 0; c < (int)S.images.size(); c++) {
+        const Image &im = S.images[c];
+        for (int k = im.lo[2]; k <= im.hi[2]; k++)
+            for (int j = im.lo[1]; j <= im.hi[1]; j++)
+                for (int i = im.lo[0]; i <= im.hi[0]; i++) S.cov_img[fillp[eidx(i, j, k)]++] = c;
+    }
+}
+
+}  // namespace
+
+Status build_system(const std::string &dir, const std::string &name, const SetupOptions &opt, System &S)
+{
+    S = System();
+    Status st = parse_inpt(dir + "/" + name + ".inpt", S);
+    if (!st.ok()) return st;
+    if (opt.nqp_override > 0) S.nqp = opt.nqp_override;
+    if (!(st = parse_ion(dir + "/" + name + ".ion", S)).ok()) return st;
+    S.psd.resize(S.ntype);
+    for (int at = 0; at < S.ntype; at++) {
+        if (!(st = parse_pot(dir + "/pseudopotential/" + S.psdFile[at], S.atomType[at], S.localPsd[at], S.psd[at])).ok())
+            return st;
+        S.psd[at].valence = S.noe[at];
+        if (!(st = fit_pseudopotential(S, S.psd[at])).ok()) return st;
+        if (!(st = pseudocharge_cutoff(S, at)).ok()) return st;
+    }
+    // Supercell list (Create_SupercellAtomList, lsdft_initObjs.cc:3031-3041) and
+    // alpha slots (Calculate_NonlocalIndex, lsdft_initObjs.cc:875-967).
+    S.scpos = S.atompos;
+    S.sctype = S.atomtype;
+    if (opt.phantom_atom) {
+        if (S.ntype != 1) return fail(4, "the phantom atom of the reference exists for NTYPE 1 only");
+        for (int d = 0; d < 3; d++) S.scpos.push_back(opt.phantom_pos[d]);
+        S.sctype.push_back(0);
+    }
+    S.ip_index.assign(S.sctype.size(), 0);
+    S.ip_length = 0;
+    for (int at = 0; at < S.ntype; at++)
+        for (size_t p = 0; p < S.sctype.size(); p++)
+            if (S.sctype[p] == at) { S.ip_index[p] = S.ip_length; S.ip_length += S.psd[at].nlm(); }
+    // rank box and its extension by the cube half-width
+    if (opt.rank_box[3] > 0) {
+        for (int d = 0; d < 3; d++) {
+            if (opt.rank_box[d] < 0 || opt.rank_box[3 + d] < 1 || opt.rank_box[d] + opt.rank_box[3 + d] > S.npts[d])
+                return fail(4, "rank box outside the grid");
+            S.box[d] = opt.rank_box[d];
+            S.box[3 + d] = opt.rank_box[3 + d];
+        }
+    } else {
+        for (int d = 0; d < 3; d++) { S.box[d] = 0; S.box[3 + d] = S.npts[d]; }
+    }
+    for (int d = 0; d < 3; d++) {
+        S.ext0[d] = S.box[d] - S.hw[d];
+        S.extn[d] = S.box[3 + d] + 2 * S.hw[d];
+    }
+    collect_images(S);
+    if (!(st = fill_projectors(S)).ok()) return st;
+    build_coverage(S);
+    return {};
+}
+
+}  // namespace sqamd
