@@ -1,0 +1,137 @@
+"""SQ grid-nodes/s of the per-node Lanczos hot path on one MI355X.
+
+One step = one SQ pass (N_qp+1 H applications + QL per node) over every grid
+node of the 16-atom hcp Mg cell (25x21x40 = 21000 nodes, 29^3 cubes, 80
+Lanczos steps, KB nonlocal projectors, V_eff = 0.333 as the reference),
+inputs resident in HBM.  Prints one JSON line.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config C16]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, REPO)
+
+METRIC = "SQ grid-nodes/sec + SCF wall-time at <=1e-5 Ha/atom, chosen vs plain FP64, 1 GPU"
+HBM_PEAK_GBS = 8000.0
+
+
+def algorithmic_bytes_per_node(nqp: int, nc: int, s: int) -> float:
+    """m*3*n_c*s + 8*n_c: two vector reads and one write per Lanczos step, one
+    V_eff cube read per node; projector tables are cache-resident."""
+    return (nqp + 1) * 3 * nc * s + 8 * nc
+
+
+def cpu_baseline(deck_dir, name, n_sample, threads):
+    """The oracle (reference-faithful FP64 C restatement) on a bounded sample of nodes."""
+    from oracle import oracle as O
+    O.build()
+    S = O.System(deck_dir, name)
+    S.build_nonlocal()
+    total = S.box[3] * S.box[4] * S.box[5]
+    ids = np.linspace(0, total - 1, n_sample).astype(np.int32)
+    t0 = time.perf_counter()
+    S.nodes_and_weights(mode=O.MODE_KB, node_ids=ids, nthreads=threads)
+    dt = time.perf_counter() - t0
+    return dict(value=n_sample / dt, unit="grid-nodes/s", cores=threads, kind="port",
+                sample=f"{n_sample} evenly spaced nodes of the same deck, KB mode, FP64, oracle/sq_oracle.c "
+                       f"(per-node operator rebuild, AIJ SpMV, projector scan over all images, full-z QL), "
+                       f"{dt:.1f} s on {threads} threads")
+
+
+def load_traffic(tag):
+    path = os.path.join(REPO, "profiles", f"{tag}_pmc.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f).get("hbm_bytes_per_launch")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", default="C16")
+    ap.add_argument("--mode", choices=["kb", "ref"], default="kb")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-sample", type=int, default=192)
+    ap.add_argument("--fp32-steps", type=int, default=None)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: every box here has one MI355X, and the SQ node loop shards across "
+              "GPUs by node blocks without collectives inside the pass (see DESIGN.md).", file=sys.stderr)
+        sys.exit(2)
+
+    import sqminiapp_amd as sq
+    from sqminiapp_amd import decks
+
+    root = tempfile.mkdtemp(prefix="sq_bench_")
+    d, name = decks.deck(args.config, root)
+    mode = sq.SQ_NL_KB if args.mode == "kb" else sq.SQ_NL_REF_COLLAPSED
+    ctx = sq.Context(d, name, device=0)
+    I = ctx.info
+    n_nodes = I.n_nodes
+
+    def timed(precision, steps, warmup):
+        for _ in range(warmup):
+            ctx.run(mode, precision)
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        kernel_ms = []
+        for _ in range(steps):
+            ctx.run(mode, precision)
+            kernel_ms.append(ctx.timing())
+        ctx.synchronize()
+        dt = time.perf_counter() - t0
+        return dt, np.array(kernel_ms)
+
+    dt64, k64 = timed(sq.SQ_FP64, args.steps, args.warmup)
+    lf, rho, eb, mts = ctx.density_fermi_energy()
+    fp32_steps = args.steps if args.fp32_steps is None else args.fp32_steps
+    dt32, k32 = timed(sq.SQ_FP32, fp32_steps, 1) if fp32_steps > 0 else (None, None)
+
+    value = n_nodes * args.steps / dt64
+    lanczos_ms = float(k64[:, 0].mean())
+    bytes_launch = algorithmic_bytes_per_node(I.nqp, I.nc, 8) * n_nodes
+    achieved = bytes_launch / (lanczos_ms * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": load_traffic(f"lanczos_{args.config.lower()}_fp64"),
+                "kernel": "lanczos_kernel<double,6>", "kernel_ms": round(lanczos_ms, 3),
+                "algorithmic_bytes_per_launch": bytes_launch}
+    out = {
+        "metric": METRIC, "value": round(value, 1), "unit": "grid-nodes/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(dt64 * 1e3 / args.steps, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"{args.config}: {I.nAtoms}-atom hcp Mg cell, grid {I.npts[0]}x{I.npts[1]}x"
+                               f"{I.npts[2]} = {n_nodes} nodes, {I.nloc[0]}^3 cubes, {I.nqp} Lanczos steps, "
+                               f"{args.mode} nonlocal, V_eff 0.333",
+                   "nodes": n_nodes, "nqp": I.nqp, "cube": I.nloc[0], "form": "plain FP64"},
+        "roofline": roofline,
+        "ql_ms": round(float(k64[:, 1].mean()), 3),
+        "fermi_level": lf, "band_energy": eb, "minus_ts": mts,
+    }
+    if dt32 is not None:
+        v32 = n_nodes * fp32_steps / dt32
+        b32 = algorithmic_bytes_per_node(I.nqp, I.nc, 4) * n_nodes
+        out["fp32_form"] = {"value": round(v32, 1), "unit": "grid-nodes/s", "speedup_vs_fp64": round(v32 / value, 3),
+                            "lanczos_ms": round(float(k32[:, 0].mean()), 3),
+                            "achieved_GBs": round(b32 / (float(k32[:, 0].mean()) * 1e-3) / 1e9, 1)}
+    if not args.no_cpu_baseline:
+        threads = max(1, min(16, os.cpu_count() or 1))
+        out["cpu_baseline"] = cpu_baseline(d, name, args.cpu_sample, threads)
+    ctx.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -140,6 +140,9 @@ int sq_tridiag_nodes_weights(sq_ctx *ctx, const double *diag, const double *subd
 int sq_density_fermi_energy(sq_ctx *ctx, double nelec, double beta, double *lambda_f, double *rho,
                             double *e_band, double *minus_ts);
 
+/* Wait for all device work of the context's device (hipDeviceSynchronize). */
+int sq_synchronize(sq_ctx *ctx);
+
 /* Projector image c of the setup: center (3), box (xs,ys,zs,xe,ye,ze),
  * supercell atom index, type, entries (Nlm*box points), and the flattened
  * chi (k,j,i outer, (l,m) inner, as lsdft_nl.cc:81-123) if chi != NULL. */

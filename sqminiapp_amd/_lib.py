@@ -24,7 +24,7 @@ STATUS = {0: "SQ_OK", 1: "SQ_ERR_IO", 2: "SQ_ERR_FORMAT", 3: "SQ_ERR_RANGE", 4: 
 # every symbol include/sq_amd.h declares
 EXPORTS = ["sq_last_error", "sq_default_options", "sq_create_from_deck", "sq_destroy", "sq_get_info", "sq_set_veff",
            "sq_run", "sq_get_nodes_and_weights", "sq_get_lanczos", "sq_nodes_and_weights", "sq_last_timing",
-           "sq_apply_h", "sq_tridiag_nodes_weights", "sq_density_fermi_energy", "sq_get_image"]
+           "sq_apply_h", "sq_tridiag_nodes_weights", "sq_density_fermi_energy", "sq_synchronize", "sq_get_image"]
 
 
 class SQError(RuntimeError):
@@ -76,6 +76,7 @@ def lib():
     L.sq_apply_h.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, dp, dp]
     L.sq_tridiag_nodes_weights.argtypes = [vp, dp, dp, C.c_int, lg, dp, dp, dp, dp]
     L.sq_density_fermi_energy.argtypes = [vp, C.c_double, C.c_double, dp, dp, dp, dp]
+    L.sq_synchronize.argtypes = [vp]
     L.sq_get_image.argtypes = [vp, C.c_int, dp, ip, ip, ip, ip, dp]
     _lib = L
     return L
@@ -175,6 +176,9 @@ class Context:
     def nodes_and_weights(self, mode=SQ_NL_REF_COLLAPSED, precision=SQ_FP64, begin=0, count=None):
         self.run(mode, precision, begin, count)
         return self.results()
+
+    def synchronize(self):
+        _check(lib().sq_synchronize(self._h))
 
     def timing(self):
         t1, t2 = C.c_double(), C.c_double()

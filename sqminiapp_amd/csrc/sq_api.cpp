@@ -608,6 +608,15 @@ int sq_density_fermi_energy(sq_ctx *c, double nelec, double beta, double *lambda
     return SQ_OK;
 }
 
+int sq_synchronize(sq_ctx *c)
+{
+    if (!c) return set_err(SQ_ERR_ARG, "null context");
+    if (!c->stream) return SQ_OK;
+    HIP_TRY(hipSetDevice(c->device));
+    HIP_TRY(hipDeviceSynchronize());
+    return SQ_OK;
+}
+
 int sq_get_image(sq_ctx *c, int g, double *center, int *box, int *atom, int *type, int *entries, double *chi)
 {
     if (!c) return set_err(SQ_ERR_ARG, "null context");
