@@ -71,6 +71,8 @@ typedef struct {
     double gamma[8][4];      /* Gamma_l per type, lsdft_initObjs.cc:822-869 */
     double cutoff[8][3];     /* pseudocharge cutoff, lsdft_initObjs.cc:628-817 */
     long long chi_entries;   /* total projector entries over all images */
+    int kernel;              /* Lanczos kernel in use: 1 column-marching, 0 generic */
+    int kmax;                /* most images meeting one (x,y) column of a cube */
 } sq_info;
 
 /* Message of the last failing call on this thread ("" if none). */

@@ -46,7 +46,7 @@ class Info(C.Structure):
         ("nc", C.c_int), ("rank_box", C.c_int * 6), ("ext_origin", C.c_int * 3), ("ext_dims", C.c_int * 3),
         ("range", C.c_double * 3), ("delta", C.c_double * 3), ("dV", C.c_double), ("beta", C.c_double),
         ("rcut", C.c_double), ("coeffs", (C.c_double * 11) * 3), ("gamma", (C.c_double * 4) * 8),
-        ("cutoff", (C.c_double * 3) * 8), ("chi_entries", C.c_longlong),
+        ("cutoff", (C.c_double * 3) * 8), ("chi_entries", C.c_longlong), ("kernel", C.c_int), ("kmax", C.c_int),
     ]
 
 

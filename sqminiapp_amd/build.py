@@ -14,7 +14,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libsqamd.so")
 ARCH = os.environ.get("SQ_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["sq_setup.cpp", "sq_api.cpp", "sq_kernels.hip"]
+SOURCES = ["sq_setup.cpp", "sq_api.cpp", "sq_kernels.hip", "sq_zmarch.hip"]
 
 # -ffp-contract=off: products and sums round as written (the FP64 form follows
 # the reference's operation order; the FP32 form asks for fma explicitly).

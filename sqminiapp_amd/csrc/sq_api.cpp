@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -40,18 +41,27 @@ struct sq_ctx {
     int device = 0, n_cu = 0, grid = 0, img_cap = 0, nlm_tmpl = 6;
     size_t lds = 0;
     int lds_A = 0, lds_alpha = 0, lds_red = 0, lds_local = 0;
+    // z-march kernel: eligibility, column-list length, LDS layout, grid per precision
+    bool zmarch = false;
+    int kmax = 0, zgrid[2] = {0, 0};
+    size_t zlds = 0;
+    int z_tlo = 0, z_tdim = 0, z_toff = 0, z_tcen = 0, z_A = 0, z_alpha = 0, z_red = 0, z_plane = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     // device tables
     double *chi64 = nullptr;
     float *chi32 = nullptr;
-    int4 *img_lo = nullptr, *img_dim = nullptr;
+    int4 *img_lo = nullptr, *img_dim = nullptr, *img_dimp = nullptr;
     int64_t *img_off = nullptr;
     int *img_atom = nullptr, *img_type = nullptr;
     double *gamma = nullptr;
     int *cov_off = nullptr, *cov_img = nullptr;
     double *diag64 = nullptr;
     float *diag32 = nullptr;
+    double *rad64 = nullptr;
+    float *rad32 = nullptr;
+    int64_t *img_rad = nullptr;
+    double4 *img_cen = nullptr;
     double *c64 = nullptr;  // 3 x (kMaxOrder+1)
     float *c32 = nullptr;
     void *scratch = nullptr;
@@ -77,7 +87,7 @@ int dev_upload(T **dst, const T *src, size_t n)
 void free_ctx(sq_ctx *c)
 {
     if (!c) return;
-    void *ptrs[] = {c->chi64, c->chi32, c->img_lo, c->img_dim, c->img_off, c->img_atom, c->img_type, c->gamma,
+    void *ptrs[] = {c->img_dimp, c->rad64, c->rad32, c->img_rad, c->img_cen, c->chi64, c->chi32, c->img_lo, c->img_dim, c->img_off, c->img_atom, c->img_type, c->gamma,
                     c->cov_off, c->cov_img, c->diag64, c->diag32, c->c64, c->c32, c->scratch, c->a, c->b,
                     c->lam, c->w, c->lmin, c->lmax, c->qlws, c->status, c->partial, c->partial2, c->out};
     for (void *p : ptrs)
@@ -123,6 +133,25 @@ int max_images_per_cube(const System &S)
     for (int k = 0; k < lz; k++)
         for (int j = 0; j < ly; j++)
             for (int i = 0; i < lx; i++) mx = std::max(mx, at(i, j, k));
+    return mx;
+}
+
+// Largest number of images any (x,y) column of any cube meets: per image, the
+// columns of its x/y box, for every node z whose cube z-range meets its box.
+int max_images_per_column(const System &S)
+{
+    const int ex = S.extn[0], ey = S.extn[1], lz = S.box[5];
+    std::vector<int> cnt((size_t)ex * ey * lz, 0);
+    int mx = 0;
+    for (const Image &im : S.images) {
+        const int k0 = std::max(im.lo[2] - S.hw[2], S.box[2]), k1 = std::min(im.hi[2] + S.hw[2], S.box[2] + lz - 1);
+        for (int k = k0; k <= k1; k++)
+            for (int j = im.lo[1]; j <= im.hi[1]; j++)
+                for (int i = im.lo[0]; i <= im.hi[0]; i++) {
+                    int &v = cnt[((size_t)(k - S.box[2]) * ey + (j - S.ext0[1])) * ex + (i - S.ext0[0])];
+                    mx = std::max(mx, ++v);
+                }
+    }
     return mx;
 }
 
@@ -186,6 +215,20 @@ int create_device(sq_ctx *c, int device)
     if ((rc = dev_upload(&c->chi64, S.chi.data(), S.chi.size()))) return rc;
     std::vector<float> chi32(S.chi.begin(), S.chi.end());
     if ((rc = dev_upload(&c->chi32, chi32.data(), chi32.size()))) return rc;
+    if ((rc = dev_upload(&c->rad64, S.rad.data(), S.rad.size()))) return rc;
+    std::vector<float> rad32(S.rad.begin(), S.rad.end());
+    if ((rc = dev_upload(&c->rad32, rad32.data(), rad32.size()))) return rc;
+    std::vector<int64_t> roff(M);
+    std::vector<double4> cen(M);
+    for (int g = 0; g < M; g++) {
+        roff[g] = S.images[g].rad_off;
+        cen[g] = make_double4(S.images[g].X0, S.images[g].Y0, S.images[g].Z0, 0.0);
+        // dims.w packs channels (8 bits), nonlocal l count (8 bits) and type (8 bits)
+        dim[g].w = S.images[g].nlm | (S.images[g].nl << 8) | (S.images[g].type << 16);
+    }
+    if ((rc = dev_upload(&c->img_dimp, dim.data(), M))) return rc;
+    if ((rc = dev_upload(&c->img_rad, roff.data(), M))) return rc;
+    if ((rc = dev_upload(&c->img_cen, cen.data(), M))) return rc;
     std::vector<double> gam((size_t)kMaxTypesDev * kMaxChannels, 0.0);
     for (int t = 0; t < S.ntype; t++)
         for (int ch = 0; ch < S.psd[t].nlm(); ch++) gam[t * kMaxChannels + ch] = S.psd[t].gamma[S.psd[t].ch_l[ch]];
@@ -234,11 +277,36 @@ int create_device(sq_ctx *c, int device)
     if (c->img_cap > 32767) return set_err(SQ_ERR_ARG, "too many images per cube");
     // one resident workgroup per CU slot; the grid strides over nodes
     c->grid = c->n_cu * 2;
-    HIP_TRY(hipMalloc(&c->scratch, (size_t)c->grid * 3 * S.nc() * sizeof(double)));
+    // z-march kernel: FD order 12, columns fit a workgroup, z and image counts
+    // fit the packed column entries, chi offsets fit 31 bits
+    const int ncol = S.nloc[0] * S.nloc[1];
+    c->kmax = max_images_per_column(S);
+    c->zmarch = S.order == 6 && ncol <= 1024 && c->img_cap <= 256 && getenv("SQ_FORCE_GENERIC") == nullptr;
+    if (c->zmarch) {
+        size_t z = align16((size_t)c->img_cap * sizeof(int));
+        c->z_tlo = (int)z;
+        z = align16(z + (size_t)c->img_cap * sizeof(int4));
+        c->z_tdim = (int)z;
+        z = align16(z + (size_t)c->img_cap * sizeof(int4));
+        c->z_toff = (int)z;
+        z = align16(z + (size_t)c->img_cap * sizeof(int64_t));
+        c->z_tcen = (int)z;
+        z = align16(z + (size_t)c->img_cap * sizeof(double4));
+        c->z_A = (int)z;
+        z = align16(z + (size_t)c->img_cap * c->nlm_tmpl * sizeof(double));
+        c->z_alpha = (int)z;
+        z = align16(z + (size_t)c->img_cap * c->nlm_tmpl * sizeof(double));
+        c->z_red = (int)z;
+        z = align16(z + 32 * sizeof(double));
+        c->z_plane = (int)z;
+        z = align16(z + 2 * (size_t)ncol * sizeof(double));
+        c->zlds = z;
+        if (c->zlds > 150 * 1024) c->zmarch = false;
+    }
     return SQ_OK;
 }
 
-KParams make_params(const sq_ctx *c, int mode, int precision, long begin, long count)
+KParams make_params(const sq_ctx *c, int mode, int precision, long begin, long count, bool zmarch = false)
 {
     const System &S = c->sys;
     KParams P;
@@ -283,11 +351,72 @@ KParams make_params(const sq_ctx *c, int mode, int precision, long begin, long c
     P.gamma_ch = c->gamma;
     P.cov_off = c->cov_off;
     P.cov_img = c->cov_img;
+    for (int d = 0; d < 3; d++)
+        for (int l = 0; l < 7 && l <= S.order; l++) {
+            P.cw64[d][l] = S.coeff[d][l];
+            P.cw32[d][l] = (float)S.coeff[d][l];
+        }
+    P.rad = precision == SQ_FP64 ? (const void *)c->rad64 : (const void *)c->rad32;
+    P.img_rad = c->img_rad;
+    P.img_cen = c->img_cen;
+    for (int d = 0; d < 3; d++) {
+        P.hgrid[d] = S.h[d];
+        P.range[d] = S.range[d];
+    }
+    for (int t = 0; t < S.ntype && t < kMaxTypesDev; t++) {
+        const Pseudopotential &ps = S.psd[t];
+        int rl[4] = {-1, -1, -1, -1}, k = 0;
+        for (int l = 0; l <= ps.lmax; l++)
+            if (l != ps.lloc) rl[l] = k++;
+        for (int ch = 0; ch < ps.nlm() && ch < kMaxChannels; ch++) {
+            P.ch_l[t][ch] = (signed char)ps.ch_l[ch];
+            P.ch_m[t][ch] = (signed char)ps.ch_m[ch];
+            P.ch_r[t][ch] = (signed char)rl[ps.ch_l[ch]];
+        }
+    }
+    P.layout = 1;
+    for (int t = 0; t < S.ntype; t++) {
+        const Pseudopotential &ps = S.psd[t];
+        const bool sd = ps.nlm() == 6 && ps.ch_l[0] == 0 && ps.ch_l[1] == 2 && ps.ch_l[5] == 2;
+        if (!sd) P.layout = 0;
+    }
+    if (zmarch) {
+        P.img_dim = c->img_dimp;
+        P.lds_A = c->z_A;
+        P.lds_alpha = c->z_alpha;
+        P.lds_red = c->z_red;
+        P.lds_tlo = c->z_tlo;
+        P.lds_tdim = c->z_tdim;
+        P.lds_toff = c->z_toff;
+        P.lds_tcen = c->z_tcen;
+        P.lds_plane = c->z_plane;
+        P.kmax = c->kmax;
+        P.xcd_map = 1;
+    }
     P.scratch = c->scratch;
     P.out_a = c->a;
     P.out_b = c->b;
     P.status = c->status;
     return P;
+}
+
+// Grids of the z-march kernel (resident workgroups, a multiple of 8 so the
+// XCD split is even) and the Lanczos scratch sized for the largest grid.
+int finish_device_setup(sq_ctx *c)
+{
+    const System &S = c->sys;
+    int maxgrid = c->grid;
+    if (c->zmarch) {
+        for (int prec = 0; prec < 2; prec++) {
+            KParams P = make_params(c, SQ_NL_KB, prec, 0, 1, true);
+            int occ = zmarch_occupancy(P, prec, c->zlds);
+            if (occ <= 0) { c->zmarch = false; break; }
+            c->zgrid[prec] = std::max(8, (c->n_cu * occ) & ~7);
+            maxgrid = std::max(maxgrid, c->zgrid[prec]);
+        }
+    }
+    HIP_TRY(hipMalloc(&c->scratch, (size_t)maxgrid * 3 * S.nc() * sizeof(double)));
+    return SQ_OK;
 }
 
 int check_status(sq_ctx *c, long begin, long count, const char *what)
@@ -339,6 +468,7 @@ int sq_create_from_deck(const char *dir, const char *name, const sq_options *opt
         return set_err(SQ_ERR_ARG, "LANCZOS_DEGREE above the supported maximum");
     }
     int rc = opt->device >= 0 ? create_device(c, opt->device) : SQ_OK;
+    if (!rc && opt->device >= 0) rc = finish_device_setup(c);
     if (rc) {
         std::string keep = g_err;
         free_ctx(c);
@@ -392,6 +522,8 @@ int sq_get_info(sq_ctx *c, sq_info *I)
         for (int d = 0; d < 3; d++) I->cutoff[t][d] = S.cutoff[t][d];
     }
     I->chi_entries = (long long)S.chi.size();
+    I->kernel = c->zmarch ? 1 : 0;
+    I->kmax = c->kmax;
     return SQ_OK;
 }
 
@@ -413,10 +545,17 @@ int sq_run(sq_ctx *c, int mode, int precision, long begin, long count)
     if (begin < 0 || count <= 0 || begin + count > S.n_nodes()) return set_err(SQ_ERR_ARG, "node range outside the rank box");
     HIP_TRY(hipSetDevice(c->device));
     c->have_run = false;
-    KParams P = make_params(c, mode, precision, begin, count);
-    const int grid = (int)std::min<long>(c->grid, count);
     HIP_TRY(hipEventRecord(c->ev[0], c->stream));
-    HIP_TRY(launch_lanczos(P, precision, grid, c->lds, c->stream));
+    if (c->zmarch) {
+        KParams P = make_params(c, mode, precision, begin, count, true);
+        int grid = c->zgrid[precision];
+        if (count < grid) grid = std::max<long>(8, (count + 7) & ~7L);
+        HIP_TRY(launch_zmarch(P, precision, grid, c->zlds, c->stream));
+    } else {
+        KParams P = make_params(c, mode, precision, begin, count);
+        const int grid = (int)std::min<long>(c->grid, count);
+        HIP_TRY(launch_lanczos(P, precision, grid, c->lds, c->stream));
+    }
     HIP_TRY(hipEventRecord(c->ev[1], c->stream));
     HIP_TRY(hipEventSynchronize(c->ev[1]));
     int rc = check_status(c, begin, count, "Lanczos");

@@ -20,6 +20,18 @@ struct KParams {
     int nqp, mode, nlm_max, n_images, img_cap;
     // LDS layout (byte offsets)
     int lds_A, lds_alpha, lds_red, lds_local;
+    int lds_tlo, lds_tdim, lds_toff, lds_tcen, lds_plane;  // z-march kernel tables and plane
+    int kmax;                                     // images per cube column (diagnostic)
+    double cw64[3][7];                            // z-march stencil weights (x,y,z) by value: scalar registers
+    float cw32[3][7];
+    // factored projectors of the z-march kernel: chi_lm = T_l * C_lm P_lm(x,y,z)
+    const void *rad;                              // radial factors (element type = precision)
+    const int64_t *img_rad;                       // first radial factor per image
+    const double4 *img_cen;                       // image position (X0,Y0,Z0,-)
+    double hgrid[3], range[3];                    // x = h*gx - R - X0 (lsdft_nl.cc:174-190)
+    signed char ch_l[kMaxTypesDev][kMaxChannels], ch_m[kMaxTypesDev][kMaxChannels], ch_r[kMaxTypesDev][kMaxChannels];
+    int xcd_map;                                  // 1: nodes of one pass split contiguously per XCD
+    int layout;                                   // 1: every type has nonlocal l = {0, 2} (s+d channels)
     // stencil weights cx[0..order] per axis (element type = precision)
     const void *cx_ptr, *cy_ptr, *cz_ptr;
     // cube diagonal fl(cdiag + V_eff) on the extended box (element type = precision)
@@ -39,6 +51,9 @@ struct KParams {
 };
 
 hipError_t launch_lanczos(const KParams &P, int precision, int grid, size_t lds, hipStream_t st);
+hipError_t launch_zmarch(const KParams &P, int precision, int grid, size_t lds, hipStream_t st);
+int zmarch_workgroup(int ncol);
+int zmarch_occupancy(const KParams &P, int precision, size_t lds);
 hipError_t launch_apply(const KParams &P, long id, const double *x, double *y, size_t lds, hipStream_t st);
 hipError_t launch_ql(const double *a, const double *b, int n, int stride, long count, double *lam, double *w,
                      double *lmin, double *lmax, int *status, double *ws, hipStream_t st);

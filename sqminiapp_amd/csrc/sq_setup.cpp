@@ -492,6 +492,16 @@ Status fill_projectors(System &S)
         S.max_nlm = std::max(S.max_nlm, im.nlm);
     }
     S.chi.assign((size_t)total, 0.0);
+    int64_t rtotal = 0;
+    for (Image &im : S.images) {
+        const Pseudopotential &P = S.psd[im.type];
+        im.nl = 0;
+        for (int l = 0; l <= P.lmax; l++)
+            if (l != P.lloc) im.nl++;
+        im.rad_off = rtotal;
+        rtotal += im.points() * im.nl;
+    }
+    S.rad.assign((size_t)rtotal, 0.0);
     std::atomic<int> next{0}, bad{0};
     auto worker = [&]() {
         for (;;) {
@@ -500,6 +510,7 @@ Status fill_projectors(System &S)
             const Image &im = S.images[c];
             const Pseudopotential &P = S.psd[im.type];
             double *out = S.chi.data() + im.chi_off;
+            double *rout = S.rad.data() + im.rad_off;
             for (int k = im.lo[2]; k <= im.hi[2]; k++) {
                 double z = S.h[2] * k - S.range[2] - im.Z0, zz = z * z;
                 for (int j = im.lo[1]; j <= im.hi[1]; j++) {
@@ -517,6 +528,18 @@ Status fill_projectors(System &S)
                             double sh = real_spherical_harmonic(x, y, z, P.ch_l[ch], P.ch_m[ch], &b);
                             if (b) { bad = 4; return; }
                             *out++ = radial[P.ch_l[ch]] * sh;
+                        }
+                        for (int l = 0; l <= P.lmax; l++) {
+                            if (l == P.lloc) continue;
+                            double t;
+                            if (l == 0) t = radial[0] * 0.282094791773878;
+                            else if (!(r >= 1e-10)) t = 0.0;
+                            else {
+                                double rl = r;
+                                for (int q = 1; q < l; q++) rl *= r;
+                                t = radial[l] / rl;
+                            }
+                            *rout++ = t;
                         }
                     }
                 }

@@ -45,7 +45,8 @@ struct Image {
     double X0 = 0, Y0 = 0, Z0 = 0;    // image position
     int lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};  // box, inclusive, global unwrapped indices
     int64_t chi_off = 0;              // first entry in System::chi
-    int nlm = 0;
+    int64_t rad_off = 0;              // first entry in System::rad
+    int nlm = 0, nl = 0;              // (l,m) channels; nonlocal l values
     int dim(int d) const { return hi[d] - lo[d] + 1; }
     int64_t points() const { return (int64_t)dim(0) * dim(1) * dim(2); }
 };
@@ -82,6 +83,11 @@ struct System {
     // ---- projector images overlapping the extended box
     std::vector<Image> images;
     std::vector<double> chi;      // per image: (k,j,i) box points outer, channel inner
+    // Radial factors of the same projectors: per image and box point, one value
+    // per nonlocal l (ascending): T_0 = C00 S_0(r), T_l = S_l(r)/r^l (l >= 1,
+    // 0 at r < 1e-10), so chi_lm = T_l C_lm P_lm(x,y,z) with P_lm the
+    // homogeneous polynomial of the real harmonic.
+    std::vector<double> rad;
     // ---- per extended-box point: images whose box contains it, ascending
     std::vector<int32_t> cov_off, cov_img;
     int max_cov = 0, max_nlm = 0;

@@ -1,0 +1,532 @@
+// sq_zmarch.hip -- the fast Lanczos kernel: one workgroup per node cube; each
+// thread owns CPT (x,y) columns of the cube and marches them along z.
+//
+// Per H application:
+//  * gather: one wavefront per projector image; lanes take the (x,y) columns
+//    of the image box inside the cube and walk z, U planes per trip, so the
+//    loads of v and of the projector radial factors are independent and in
+//    flight together;
+//  * alpha: per atom from the gathered sums (collapsed or KB slots);
+//  * apply: the 12 z-neighbours of the stencil come from a 13-deep register
+//    window of each column, the 24 x/y-neighbours from the current z-plane
+//    staged in LDS (double-buffered, one barrier per plane); neighbours outside
+//    the cube enter as exact zeros, so the sum runs over the reference's
+//    ascending AIJ column order with every product and sum rounded as written;
+//  * nonlocal: each column then adds chi*alpha of the images covering it (a
+//    register bitmask over the node's image list, built once per node), in
+//    ascending image order and channel order -- after the stencil, as the
+//    reference adds Mult_NonlocalVector after MatMult -- and forms its part of
+//    <v,u>; chi_lm = T_l * C_lm P_lm(x,y,z) is rebuilt from the radial factor;
+//    the projector sums use fused multiply-adds (their order already differs
+//    from the reference's sequential scan), the stencil does not;
+//  * update: w = (u - a v) - b vprev and ||w|| with the same column mapping.
+// 512 threads leave each lane 256 registers, which the two columns' windows,
+// the projector registers and the load batches need.
+#include <type_traits>
+
+#include "sq_device_common.h"
+
+namespace sqamd {
+
+template <typename T, int NLM, int CPT, int MW, int LAYOUT>
+__global__ void __launch_bounds__(512) lanczos_zmarch(KParams P)
+{
+    constexpr int WG = 512;
+    constexpr int NW = WG / 64;
+    constexpr int O = 6;
+    constexpr int U = 8;  // z-planes per trip of the gather and scatter loops
+    extern __shared__ __align__(16) unsigned char smem[];
+    int *img_list = reinterpret_cast<int *>(smem);
+    int4 *tlo = reinterpret_cast<int4 *>(smem + P.lds_tlo);
+    int4 *tdim = reinterpret_cast<int4 *>(smem + P.lds_tdim);
+    int64_t *toff = reinterpret_cast<int64_t *>(smem + P.lds_toff);   // radial-factor offset per image
+    double4 *tcen = reinterpret_cast<double4 *>(smem + P.lds_tcen);   // image position
+    double *A = reinterpret_cast<double *>(smem + P.lds_A);
+    double *alpha = reinterpret_cast<double *>(smem + P.lds_alpha);
+    double *red = reinterpret_cast<double *>(smem + P.lds_red);
+    T *plane = reinterpret_cast<T *>(smem + P.lds_plane);
+    __shared__ int s_count;
+    __shared__ T s_w[3][8];
+
+    const int nx = P.nloc[0], ny = P.nloc[1], nz = P.nloc[2];
+    const int ncol = nx * ny, nc = P.nc;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    int col[CPT], ci[CPT], cj[CPT];
+    bool has[CPT];
+#pragma unroll
+    for (int k = 0; k < CPT; k++) {
+        col[k] = tid + k * WG;
+        has[k] = col[k] < ncol;
+        cj[k] = has[k] ? col[k] / nx : 0;
+        ci[k] = has[k] ? col[k] - cj[k] * nx : 0;
+    }
+    T *__restrict__ buf0 = reinterpret_cast<T *>(P.scratch) + (size_t)blockIdx.x * 3 * nc;
+    T *__restrict__ buf1 = buf0 + nc;
+    T *__restrict__ Bu = buf1 + nc;
+    const T *__restrict__ diag_ext = reinterpret_cast<const T *>(P.diag_ext);
+    const T *__restrict__ rad = reinterpret_cast<const T *>(P.rad);
+    // stencil weights, staged through LDS into vector registers (the scalar
+    // register file is taken by the node and image bookkeeping)
+    if (tid < 21) {
+        if constexpr (sizeof(T) == 8) s_w[tid / 7][tid % 7] = P.cw64[tid / 7][tid % 7];
+        else s_w[tid / 7][tid % 7] = P.cw32[tid / 7][tid % 7];
+    }
+    __syncthreads();
+    T wreg[3][O + 1];
+#pragma unroll
+    for (int d = 0; d < 3; d++)
+#pragma unroll
+        for (int l = 0; l <= O; l++) wreg[d][l] = s_w[d][l];
+    auto wgt = [&](int d, int l) -> T { return wreg[d][l]; };
+
+    // Nodes are taken in passes of gridDim.x; within a pass the workgroups of
+    // one XCD (blockIdx % 8 under round-robin dispatch) take a contiguous run,
+    // so the projector tables they share stay in that XCD's L2.
+    const int G = gridDim.x;
+    const int xcd = blockIdx.x & 7, slot = blockIdx.x >> 3;
+    const bool xmap = P.xcd_map && (G & 7) == 0;
+    for (long pass = 0; pass * G < P.node_count; pass++) {
+        const long r = pass * G + (xmap ? (long)xcd * (G >> 3) + slot : (long)blockIdx.x);
+        if (r >= P.node_count) continue;
+        const long id = P.node_begin + r;
+        int node[3];
+        node_coords(P, id, node);
+        const int cube_lo[3] = {node[0] - P.hw[0], node[1] - P.hw[1], node[2] - P.hw[2]};
+        // ---- images whose box meets the cube, ascending, with their tables in LDS
+        if (tid == 0) s_count = 0;
+        __syncthreads();
+        for (int base = 0; base < P.n_images; base += WG) {
+            const int g = base + tid;
+            bool hit = false;
+            int4 lo4 = make_int4(0, 0, 0, 0), dm4 = make_int4(0, 0, 0, 0);
+            if (g < P.n_images) {
+                lo4 = P.img_lo[g];
+                dm4 = P.img_dim[g];
+                hit = lo4.x <= cube_lo[0] + nx - 1 && lo4.x + dm4.x - 1 >= cube_lo[0] &&
+                      lo4.y <= cube_lo[1] + ny - 1 && lo4.y + dm4.y - 1 >= cube_lo[1] &&
+                      lo4.z <= cube_lo[2] + nz - 1 && lo4.z + dm4.z - 1 >= cube_lo[2];
+            }
+            const unsigned long long bal = __ballot(hit);
+            const int before = __popcll(bal & ((1ull << lane) - 1ull));
+            int *wcnt = reinterpret_cast<int *>(red);
+            if (lane == 0) wcnt[wave] = __popcll(bal);
+            __syncthreads();
+            int off = s_count;
+            for (int w = 0; w < wave; w++) off += wcnt[w];
+            if (hit) {
+                const int li = off + before;
+                if (li < P.img_cap) {
+                    img_list[li] = g;
+                    tlo[li] = lo4;
+                    tdim[li] = dm4;
+                    toff[li] = P.img_rad[g];
+                    tcen[li] = P.img_cen[g];
+                }
+            }
+            __syncthreads();
+            if (tid == 0) {
+                int t = s_count;
+                for (int w = 0; w < NW; w++) t += wcnt[w];
+                s_count = t;
+            }
+            __syncthreads();
+        }
+        const int nimg = s_count;
+        if (nimg > P.img_cap) {
+            if (tid == 0) P.status[id] = 4;
+            __syncthreads();
+            continue;
+        }
+        // ---- per column: bitmask of the node's images whose x/y box holds it
+        unsigned cmask[CPT][MW];
+        int gxc[CPT], gyc[CPT];
+        int64_t ecol[CPT];
+#pragma unroll
+        for (int k = 0; k < CPT; k++) {
+            gxc[k] = cube_lo[0] + ci[k];
+            gyc[k] = cube_lo[1] + cj[k];
+            ecol[k] = ((int64_t)(cube_lo[2] - P.ext0[2]) * P.extn[1] + (gyc[k] - P.ext0[1])) * P.extn[0] +
+                      (gxc[k] - P.ext0[0]);
+#pragma unroll
+            for (int w = 0; w < MW; w++) cmask[k][w] = 0u;
+        }
+        for (int li = 0; li < nimg; li++) {
+            const int4 lo = tlo[li], dm = tdim[li];
+#pragma unroll
+            for (int k = 0; k < CPT; k++) {
+                const bool in = has[k] && gxc[k] >= lo.x && gxc[k] < lo.x + dm.x && gyc[k] >= lo.y && gyc[k] < lo.y + dm.y;
+#pragma unroll
+                for (int w = 0; w < MW; w++)
+                    if (w == (li >> 5) && in) cmask[k][w] |= 1u << (li & 31);
+            }
+        }
+        const int64_t eplane = (int64_t)P.extn[1] * P.extn[0];
+        // ---- Lanczos start: v = e_center, vprev = 0
+        for (int p = tid; p < nc; p += WG) { buf0[p] = T(0); buf1[p] = T(0); }
+        __syncthreads();
+        if (tid == 0) buf0[P.center] = T(1);
+        __syncthreads();
+        T *Bv = buf0, *Bvp = buf1;
+        T sv = T(1), svp = T(0);
+        double bprev = 0.0;
+        double *out_a = P.out_a + id * (P.nqp + 1), *out_b = P.out_b + id * (P.nqp + 1);
+        int status = 0;
+        for (int step = 0; step <= P.nqp; step++) {
+            // ---- gather: A[li][ch] = sum_{box & cube} v chi
+            for (int li = wave; li < nimg; li += NW) {
+                const int4 lo = tlo[li], dm = tdim[li];
+                const int nlm = dm.w & 255, nl = (dm.w >> 8) & 255, type = dm.w >> 16;
+                const double4 cen = tcen[li];
+                int a0[3], n[3];
+                const int ilo[3] = {lo.x, lo.y, lo.z}, idm[3] = {dm.x, dm.y, dm.z};
+#pragma unroll
+                for (int d = 0; d < 3; d++) {
+                    const int s0 = max(ilo[d], cube_lo[d]);
+                    const int e0 = min(ilo[d] + idm[d] - 1, cube_lo[d] + P.nloc[d] - 1);
+                    a0[d] = s0;
+                    n[d] = e0 - s0 + 1;
+                }
+                const int nxy = n[0] * n[1];
+                const float inv0 = 1.0f / (float)n[0];
+                const int zs = dm.x * dm.y * nl;
+                T acc[NLM];
+#pragma unroll
+                for (int ch = 0; ch < NLM; ch++) acc[ch] = T(0);
+                for (int q = lane; q < nxy; q += 64) {
+                    const int yi = fdiv(q, n[0], inv0);
+                    const int xi = q - yi * n[0];
+                    const int gx = a0[0] + xi, gy = a0[1] + yi;
+                    const int p0 = ((a0[2] - cube_lo[2]) * ny + (gy - cube_lo[1])) * nx + (gx - cube_lo[0]);
+                    const int64_t rb = toff[li] + ((int64_t)((a0[2] - lo.z) * dm.y + (gy - lo.y)) * dm.x + (gx - lo.x)) * nl;
+                    const T x = (T)(P.hgrid[0] * gx - P.range[0] - cen.x);
+                    const T y = (T)(P.hgrid[1] * gy - P.range[1] - cen.y);
+                    const T *vb = Bv + p0;
+                    if constexpr (LAYOUT == 1) {
+                        // two radial factors per point, one 16-byte load
+                        using V2 = typename std::conditional<sizeof(T) == 8, double2, float2>::type;
+                        const V2 *r2 = reinterpret_cast<const V2 *>(rad + rb);
+                        const int zs2 = dm.x * dm.y;
+                        int zi = 0;
+#pragma unroll 1
+                        for (; zi + U <= n[2]; zi += U) {
+                            T v4[U];
+                            V2 t4[U];
+#pragma unroll
+                            for (int u = 0; u < U; u++) {
+                                v4[u] = vb[(zi + u) * ncol];
+                                t4[u] = r2[(zi + u) * zs2];
+                            }
+#pragma unroll
+                            for (int u = 0; u < U; u++) {
+                                const T z = (T)(P.hgrid[2] * (a0[2] + zi + u) - P.range[2] - cen.z);
+                                const T tt[2] = {t4[u].x, t4[u].y};
+                                T c[NLM];
+                                chi_channels<T, NLM, 1>(P, type, nlm, tt, x, y, z, c);
+                                const T v = sv * v4[u];
+#pragma unroll
+                                for (int ch = 0; ch < NLM; ch++) acc[ch] = fma(v, c[ch], acc[ch]);
+                            }
+                        }
+#pragma unroll 1
+                        for (; zi < n[2]; zi++) {
+                            const V2 t2 = r2[zi * zs2];
+                            const T z = (T)(P.hgrid[2] * (a0[2] + zi) - P.range[2] - cen.z);
+                            const T tt[2] = {t2.x, t2.y};
+                            T c[NLM];
+                            chi_channels<T, NLM, 1>(P, type, nlm, tt, x, y, z, c);
+                            const T v = sv * vb[zi * ncol];
+#pragma unroll
+                            for (int ch = 0; ch < NLM; ch++) acc[ch] = fma(v, c[ch], acc[ch]);
+                        }
+                    } else {
+                        const T *r0 = rad + rb;
+                        for (int zi = 0; zi < n[2]; zi += U) {
+                            T v4[U], t4[U][4];
+#pragma unroll
+                            for (int u = 0; u < U; u++) {
+                                const bool ok = zi + u < n[2];
+                                v4[u] = ok ? vb[(zi + u) * ncol] : T(0);
+#pragma unroll
+                                for (int k = 0; k < 4; k++) t4[u][k] = (ok && k < nl) ? r0[(int64_t)(zi + u) * zs + k] : T(0);
+                            }
+#pragma unroll
+                            for (int u = 0; u < U; u++) {
+                                if (zi + u < n[2]) {
+                                    const T z = (T)(P.hgrid[2] * (a0[2] + zi + u) - P.range[2] - cen.z);
+                                    T c[NLM];
+                                    chi_channels<T, NLM, LAYOUT>(P, type, nlm, t4[u], x, y, z, c);
+                                    const T v = sv * v4[u];
+#pragma unroll
+                                    for (int ch = 0; ch < NLM; ch++) acc[ch] = fma(v, c[ch], acc[ch]);
+                                }
+                            }
+                        }
+                    }
+                }
+#pragma unroll
+                for (int ch = 0; ch < NLM; ch++) {
+                    const double sm = wave_sum((double)acc[ch]);
+                    if (lane == 0 && ch < nlm) A[li * NLM + ch] = sm;
+                }
+            }
+            __syncthreads();
+            images_alpha<NLM>(P, img_list, nimg, A, alpha);
+            __syncthreads();
+            // ---- apply, marching along z
+            T win[CPT][2 * O + 1];
+#pragma unroll
+            for (int k = 0; k < CPT; k++)
+#pragma unroll
+                for (int q = 0; q <= 2 * O; q++) {
+                    const int z = q - O;
+                    win[k][q] = (has[k] && z >= 0 && z < nz) ? sv * Bv[col[k] + z * ncol] : T(0);
+                }
+            for (int z = 0; z < nz; z++) {
+                T *pl = plane + (z & 1) * ncol;
+                T next[CPT], dg[CPT];
+#pragma unroll
+                for (int k = 0; k < CPT; k++) {
+                    next[k] = T(0);
+                    dg[k] = T(0);
+                    if (has[k]) {
+                        pl[col[k]] = win[k][O];
+                        if (z + O + 1 < nz) next[k] = sv * Bv[col[k] + (z + O + 1) * ncol];
+                        dg[k] = diag_ext[ecol[k] + z * eplane];
+                    }
+                }
+                __syncthreads();
+#pragma unroll
+                for (int k = 0; k < CPT; k++) {
+                    if (has[k]) {
+                        const int c = col[k];
+                        T sum = T(0);
+                        auto add = [&](T cf, T v) {
+                            if constexpr (sizeof(T) == 8) sum = sum + cf * v;
+                            else sum = fmaf(cf, v, sum);
+                        };
+#pragma unroll
+                        for (int l = O; l >= 1; l--) add(wgt(2, l), win[k][O - l]);
+#pragma unroll
+                        for (int l = O; l >= 1; l--) add(wgt(1, l), cj[k] - l >= 0 ? pl[c - l * nx] : T(0));
+#pragma unroll
+                        for (int l = O; l >= 1; l--) add(wgt(0, l), ci[k] - l >= 0 ? pl[c - l] : T(0));
+                        add(dg[k], win[k][O]);
+#pragma unroll
+                        for (int l = 1; l <= O; l++) add(wgt(0, l), ci[k] + l < nx ? pl[c + l] : T(0));
+#pragma unroll
+                        for (int l = 1; l <= O; l++) add(wgt(1, l), cj[k] + l < ny ? pl[c + l * nx] : T(0));
+#pragma unroll
+                        for (int l = 1; l <= O; l++) add(wgt(2, l), win[k][O + l]);
+                        Bu[c + z * ncol] = sum;
+                    }
+#pragma unroll
+                    for (int q = 0; q < 2 * O; q++) win[k][q] = win[k][q + 1];
+                    win[k][2 * O] = next[k];
+                }
+            }
+            // ---- nonlocal: chi*alpha of each column's images, ascending, then <v,u>
+            double pa = 0.0;
+            CSum ca{0.f, 0.f};
+#pragma unroll
+            for (int k = 0; k < CPT; k++) {
+                if (!has[k]) continue;
+                const int c = col[k];
+#pragma unroll 1
+                for (int w = 0; w < MW; w++) {
+                    // mask word w by a select chain: no dynamic register indexing
+                    unsigned m = cmask[k][0];
+#pragma unroll
+                    for (int q = 1; q < MW; q++) m = (w == q) ? cmask[k][q] : m;
+                    while (m) {
+                        const int li = (w << 5) + __ffs(m) - 1;
+                        m &= m - 1u;
+                        const int4 lo = tlo[li], dm = tdim[li];
+                        const int nlm = dm.w & 255, nl = (dm.w >> 8) & 255, type = dm.w >> 16;
+                        const double4 cen = tcen[li];
+                        const int zlo = max(lo.z, cube_lo[2]) - cube_lo[2];
+                        const int zhi = min(lo.z + dm.z - 1, cube_lo[2] + nz - 1) - cube_lo[2];
+                        const int64_t rb = toff[li] +
+                                           ((int64_t)((cube_lo[2] + zlo - lo.z) * dm.y + (gyc[k] - lo.y)) * dm.x + (gxc[k] - lo.x)) * nl;
+                        const T x = (T)(P.hgrid[0] * gxc[k] - P.range[0] - cen.x);
+                        const T y = (T)(P.hgrid[1] * gyc[k] - P.range[1] - cen.y);
+                        T al[NLM];
+#pragma unroll
+                        for (int ch = 0; ch < NLM; ch++) al[ch] = ch < nlm ? (T)alpha[li * NLM + ch] : T(0);
+                        T *bu = Bu + c + zlo * ncol;
+                        const int nzr = zhi - zlo + 1;
+                        if constexpr (LAYOUT == 1) {
+                            using V2 = typename std::conditional<sizeof(T) == 8, double2, float2>::type;
+                            const V2 *r2 = reinterpret_cast<const V2 *>(rad + rb);
+                            const int zs2 = dm.x * dm.y;
+                            int i = 0;
+#pragma unroll 1
+                            for (; i + U <= nzr; i += U) {
+                                T s4[U];
+                                V2 t4[U];
+#pragma unroll
+                                for (int u = 0; u < U; u++) {
+                                    s4[u] = bu[(i + u) * ncol];
+                                    t4[u] = r2[(i + u) * zs2];
+                                }
+#pragma unroll
+                                for (int u = 0; u < U; u++) {
+                                    const T zc = (T)(P.hgrid[2] * (cube_lo[2] + zlo + i + u) - P.range[2] - cen.z);
+                                    const T tt[2] = {t4[u].x, t4[u].y};
+                                    T cv[NLM];
+                                    chi_channels<T, NLM, 1>(P, type, nlm, tt, x, y, zc, cv);
+                                    T sum = s4[u];
+#pragma unroll
+                                    for (int ch = 0; ch < NLM; ch++) sum = fma(cv[ch], al[ch], sum);
+                                    bu[(i + u) * ncol] = sum;
+                                }
+                            }
+#pragma unroll 1
+                            for (; i < nzr; i++) {
+                                const V2 t2 = r2[i * zs2];
+                                const T zc = (T)(P.hgrid[2] * (cube_lo[2] + zlo + i) - P.range[2] - cen.z);
+                                const T tt[2] = {t2.x, t2.y};
+                                T cv[NLM];
+                                chi_channels<T, NLM, 1>(P, type, nlm, tt, x, y, zc, cv);
+                                T sum = bu[i * ncol];
+#pragma unroll
+                                for (int ch = 0; ch < NLM; ch++) sum = fma(cv[ch], al[ch], sum);
+                                bu[i * ncol] = sum;
+                            }
+                        } else {
+                            const T *r0 = rad + rb;
+                            const int64_t zs = (int64_t)dm.x * dm.y * nl;
+#pragma unroll 1
+                            for (int i = 0; i < nzr; i++) {
+                                T tt[4];
+#pragma unroll
+                                for (int q = 0; q < 4; q++) tt[q] = q < nl ? r0[i * zs + q] : T(0);
+                                const T zc = (T)(P.hgrid[2] * (cube_lo[2] + zlo + i) - P.range[2] - cen.z);
+                                T cv[NLM];
+                                chi_channels<T, NLM, LAYOUT>(P, type, nlm, tt, x, y, zc, cv);
+                                T sum = bu[i * ncol];
+#pragma unroll
+                                for (int ch = 0; ch < NLM; ch++)
+                                    if (ch < nlm) sum = fma(cv[ch], al[ch], sum);
+                                bu[i * ncol] = sum;
+                            }
+                        }
+                    }
+                }
+                for (int z = 0; z < nz; z++) {
+                    const T vc = sv * Bv[c + z * ncol];
+                    const T u = Bu[c + z * ncol];
+                    if constexpr (sizeof(T) == 8) pa += vc * u;
+                    else ca = csum_add(ca, vc * u);
+                }
+            }
+            double a;
+            if constexpr (sizeof(T) == 8) a = block_sum<NW>(pa, red);
+            else a = block_csum<NW>(ca, red);
+            // ---- w = (u - a v) - bprev vprev over vprev's storage
+            const T ta = T(-a), tb = T(-bprev);
+            double pb = 0.0;
+            CSum cb{0.f, 0.f};
+#pragma unroll
+            for (int k = 0; k < CPT; k++) {
+                if (!has[k]) continue;
+                for (int z = 0; z < nz; z++) {
+                    const int p = col[k] + z * ncol;
+                    const T vc = sv * Bv[p];
+                    const T vpc = svp * Bvp[p];
+                    const T u = Bu[p];
+                    T w;
+                    if constexpr (sizeof(T) == 8) {
+                        w = u + ta * vc;
+                        w = w + tb * vpc;
+                        pb += w * w;
+                    } else {
+                        w = fmaf(ta, vc, u);
+                        w = fmaf(tb, vpc, w);
+                        cb = csum_add(cb, w * w);
+                    }
+                    Bvp[p] = w;
+                }
+            }
+            double bb;
+            if constexpr (sizeof(T) == 8) bb = block_sum<NW>(pb, red);
+            else bb = block_csum<NW>(cb, red);
+            const double b = sqrt(bb);
+            if (tid == 0) { out_a[step] = a; out_b[step] = b; }
+            if (b == 0.0) { status = 7; break; }
+            T *t = Bv; Bv = Bvp; Bvp = t;
+            svp = sv;
+            sv = T(1.0 / b);
+            bprev = b;
+        }
+        if (tid == 0) P.status[id] = status;
+        __syncthreads();
+    }
+}
+
+template <typename T, int NLM, int CPT, int LAYOUT>
+static hipError_t launch_z(const KParams &P, int grid, size_t lds, hipStream_t st)
+{
+    if (P.img_cap <= 128) hipLaunchKernelGGL((lanczos_zmarch<T, NLM, CPT, 4, LAYOUT>), dim3(grid), dim3(512), lds, st, P);
+    else if (P.img_cap <= 256) hipLaunchKernelGGL((lanczos_zmarch<T, NLM, CPT, 8, LAYOUT>), dim3(grid), dim3(512), lds, st, P);
+    else return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+template <typename T, int NLM, int CPT, int LAYOUT>
+static int occ_z(const KParams &P, size_t lds)
+{
+    int n = 0;
+    hipError_t e = P.img_cap <= 128
+        ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, lanczos_zmarch<T, NLM, CPT, 4, LAYOUT>, 512, lds)
+        : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, lanczos_zmarch<T, NLM, CPT, 8, LAYOUT>, 512, lds);
+    return e == hipSuccess ? n : 0;
+}
+
+// Calls f.template run<T, NLM, CPT, LAYOUT>() for the precision, the channel
+// table, the columns per thread and the channel layout of P.
+template <typename R, typename F>
+static R dispatch(const KParams &P, int precision, F &&f)
+{
+    const bool two = P.nloc[0] * P.nloc[1] > 512;
+    const bool sd = P.layout == 1 && P.nlm_max <= 6;
+    if (precision == 0) {
+        if (sd) return two ? f.template run<double, 6, 2, 1>() : f.template run<double, 6, 1, 1>();
+        if (P.nlm_max <= 6) return two ? f.template run<double, 6, 2, 0>() : f.template run<double, 6, 1, 0>();
+        return two ? f.template run<double, kMaxChannels, 2, 0>() : f.template run<double, kMaxChannels, 1, 0>();
+    }
+    if (sd) return two ? f.template run<float, 6, 2, 1>() : f.template run<float, 6, 1, 1>();
+    if (P.nlm_max <= 6) return two ? f.template run<float, 6, 2, 0>() : f.template run<float, 6, 1, 0>();
+    return two ? f.template run<float, kMaxChannels, 2, 0>() : f.template run<float, kMaxChannels, 1, 0>();
+}
+
+struct LaunchOp {
+    const KParams &P;
+    int grid;
+    size_t lds;
+    hipStream_t st;
+    template <typename T, int NLM, int CPT, int LAYOUT>
+    hipError_t run() { return launch_z<T, NLM, CPT, LAYOUT>(P, grid, lds, st); }
+};
+
+struct OccOp {
+    const KParams &P;
+    size_t lds;
+    template <typename T, int NLM, int CPT, int LAYOUT>
+    int run() { return occ_z<T, NLM, CPT, LAYOUT>(P, lds); }
+};
+
+hipError_t launch_zmarch(const KParams &P, int precision, int grid, size_t lds, hipStream_t st)
+{
+    if (P.nloc[0] * P.nloc[1] > 1024) return hipErrorInvalidValue;
+    return dispatch<hipError_t>(P, precision, LaunchOp{P, grid, lds, st});
+}
+
+int zmarch_workgroup(int) { return 512; }
+
+int zmarch_occupancy(const KParams &P, int precision, size_t lds)
+{
+    if (P.nloc[0] * P.nloc[1] > 1024) return 0;
+    return dispatch<int>(P, precision, OccOp{P, lds});
+}
+
+}  // namespace sqamd
