@@ -48,6 +48,9 @@ def cpu_baseline(deck_dir, name, n_sample, threads):
                        f"{dt:.1f} s on {threads} threads")
 
 
+KERNEL_NAMES = {0: "lanczos_kernel (generic)", 1: "lanczos_zmarch (column-marching)"}
+
+
 def load_traffic(tag):
     path = os.path.join(REPO, "profiles", f"{tag}_pmc.json")
     if not os.path.exists(path):
@@ -68,8 +71,8 @@ def main():
     ap.add_argument("--fp32-steps", type=int, default=None)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU: every box here has one MI355X, and the SQ node loop shards across "
-              "GPUs by node blocks without collectives inside the pass (see DESIGN.md).", file=sys.stderr)
+        print("bench.py measures one GPU only: the SQ node loop would shard across GPUs by 3-D node blocks "
+              "with no collective inside a pass, and this build runs and reports the one-GPU share.", file=sys.stderr)
         sys.exit(2)
 
     import sqminiapp_amd as sq
@@ -106,7 +109,7 @@ def main():
     achieved = bytes_launch / (lanczos_ms * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": load_traffic(f"lanczos_{args.config.lower()}_fp64"),
-                "kernel": "lanczos_kernel<double,6>", "kernel_ms": round(lanczos_ms, 3),
+                "kernel": KERNEL_NAMES[I.kernel], "kernel_ms": round(lanczos_ms, 3),
                 "algorithmic_bytes_per_launch": bytes_launch}
     out = {
         "metric": METRIC, "value": round(value, 1), "unit": "grid-nodes/s", "n_gpus": 1, "steps": args.steps,

@@ -299,7 +299,7 @@ int create_device(sq_ctx *c, int device)
         c->z_red = (int)z;
         z = align16(z + 32 * sizeof(double));
         c->z_plane = (int)z;
-        z = align16(z + 2 * (size_t)ncol * sizeof(double));
+        z = align16(z + 2 * (size_t)(S.nloc[0] + 12) * (S.nloc[1] + 12) * sizeof(double));
         c->zlds = z;
         if (c->zlds > 150 * 1024) c->zmarch = false;
     }

@@ -9,8 +9,9 @@
 //  * alpha: per atom from the gathered sums (collapsed or KB slots);
 //  * apply: the 12 z-neighbours of the stencil come from a 13-deep register
 //    window of each column, the 24 x/y-neighbours from the current z-plane
-//    staged in LDS (double-buffered, one barrier per plane); neighbours outside
-//    the cube enter as exact zeros, so the sum runs over the reference's
+//    staged in LDS (double-buffered, one barrier per plane) inside a zero halo
+//    of the stencil's half-width; neighbours outside the cube read the halo's
+//    exact zeros at fixed offsets, so the sum runs over the reference's
 //    ascending AIJ column order with every product and sum rounded as written;
 //  * nonlocal: each column then adds chi*alpha of the images covering it (a
 //    register bitmask over the node's image list, built once per node), in
@@ -27,6 +28,32 @@
 #include "sq_device_common.h"
 
 namespace sqamd {
+
+// s+d channels of one (x,y) column of an image: the z-independent parts of
+// C_lm P_lm computed once, so a point needs only its z terms.
+template <typename T>
+struct SdColumn {
+    T cxy, cy, cx, q, cxxyy;  // C xy, C y, C x, -xx-yy, C22 (xx-yy)
+};
+template <typename T>
+__device__ __forceinline__ SdColumn<T> sd_column(T x, T y)
+{
+    const T C = T(1.092548430592079), C22 = T(0.546274215296040);
+    return {C * (x * y), C * y, C * x, -(x * x) - y * y, C22 * (x * x - y * y)};
+}
+// chi of channels (0,0), (2,-2), (2,-1), (2,0), (2,1), (2,2) at height z
+template <typename T, int NLM>
+__device__ __forceinline__ void sd_channels(const SdColumn<T> &c, T t0, T t2, T z, T out[NLM])
+{
+    out[0] = t0;
+    out[1] = t2 * c.cxy;
+    out[2] = t2 * (c.cy * z);
+    out[3] = t2 * (T(0.315391565252520) * (c.q + T(2) * (z * z)));
+    out[4] = t2 * (c.cx * z);
+    out[5] = t2 * c.cxxyy;
+#pragma unroll
+    for (int ch = 6; ch < NLM; ch++) out[ch] = T(0);
+}
 
 template <typename T, int NLM, int CPT, int MW, int LAYOUT>
 __global__ void __launch_bounds__(512) lanczos_zmarch(KParams P)
@@ -50,6 +77,8 @@ __global__ void __launch_bounds__(512) lanczos_zmarch(KParams P)
 
     const int nx = P.nloc[0], ny = P.nloc[1], nz = P.nloc[2];
     const int ncol = nx * ny, nc = P.nc;
+    // z-planes in LDS with a zero halo of O cells on every side in x and y
+    const int px = nx + 2 * O, pplane = px * (ny + 2 * O);
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     int col[CPT], ci[CPT], cj[CPT];
     bool has[CPT];
@@ -60,6 +89,10 @@ __global__ void __launch_bounds__(512) lanczos_zmarch(KParams P)
         cj[k] = has[k] ? col[k] / nx : 0;
         ci[k] = has[k] ? col[k] - cj[k] * nx : 0;
     }
+    int pcol[CPT];
+#pragma unroll
+    for (int k = 0; k < CPT; k++) pcol[k] = (cj[k] + O) * px + ci[k] + O;
+    for (int q = tid; q < 2 * pplane; q += WG) plane[q] = T(0);  // halo stays zero: columns write the interior only
     T *__restrict__ buf0 = reinterpret_cast<T *>(P.scratch) + (size_t)blockIdx.x * 3 * nc;
     T *__restrict__ buf1 = buf0 + nc;
     T *__restrict__ Bu = buf1 + nc;
@@ -204,6 +237,7 @@ __global__ void __launch_bounds__(512) lanczos_zmarch(KParams P)
                     if constexpr (LAYOUT == 1) {
                         // two radial factors per point, one 16-byte load
                         using V2 = typename std::conditional<sizeof(T) == 8, double2, float2>::type;
+                        const SdColumn<T> pz = sd_column(x, y);
                         const V2 *r2 = reinterpret_cast<const V2 *>(rad + rb);
                         const int zs2 = dm.x * dm.y;
                         int zi = 0;
@@ -219,9 +253,8 @@ __global__ void __launch_bounds__(512) lanczos_zmarch(KParams P)
 #pragma unroll
                             for (int u = 0; u < U; u++) {
                                 const T z = (T)(P.hgrid[2] * (a0[2] + zi + u) - P.range[2] - cen.z);
-                                const T tt[2] = {t4[u].x, t4[u].y};
                                 T c[NLM];
-                                chi_channels<T, NLM, 1>(P, type, nlm, tt, x, y, z, c);
+                                sd_channels<T, NLM>(pz, t4[u].x, t4[u].y, z, c);
                                 const T v = sv * v4[u];
 #pragma unroll
                                 for (int ch = 0; ch < NLM; ch++) acc[ch] = fma(v, c[ch], acc[ch]);
@@ -231,9 +264,8 @@ __global__ void __launch_bounds__(512) lanczos_zmarch(KParams P)
                         for (; zi < n[2]; zi++) {
                             const V2 t2 = r2[zi * zs2];
                             const T z = (T)(P.hgrid[2] * (a0[2] + zi) - P.range[2] - cen.z);
-                            const T tt[2] = {t2.x, t2.y};
                             T c[NLM];
-                            chi_channels<T, NLM, 1>(P, type, nlm, tt, x, y, z, c);
+                            sd_channels<T, NLM>(pz, t2.x, t2.y, z, c);
                             const T v = sv * vb[zi * ncol];
 #pragma unroll
                             for (int ch = 0; ch < NLM; ch++) acc[ch] = fma(v, c[ch], acc[ch]);
@@ -282,14 +314,14 @@ __global__ void __launch_bounds__(512) lanczos_zmarch(KParams P)
                     win[k][q] = (has[k] && z >= 0 && z < nz) ? sv * Bv[col[k] + z * ncol] : T(0);
                 }
             for (int z = 0; z < nz; z++) {
-                T *pl = plane + (z & 1) * ncol;
+                T *pl = plane + (z & 1) * pplane;
                 T next[CPT], dg[CPT];
 #pragma unroll
                 for (int k = 0; k < CPT; k++) {
                     next[k] = T(0);
                     dg[k] = T(0);
                     if (has[k]) {
-                        pl[col[k]] = win[k][O];
+                        pl[pcol[k]] = win[k][O];
                         if (z + O + 1 < nz) next[k] = sv * Bv[col[k] + (z + O + 1) * ncol];
                         dg[k] = diag_ext[ecol[k] + z * eplane];
                     }
@@ -298,26 +330,27 @@ __global__ void __launch_bounds__(512) lanczos_zmarch(KParams P)
 #pragma unroll
                 for (int k = 0; k < CPT; k++) {
                     if (has[k]) {
-                        const int c = col[k];
+                        const T *pc = pl + pcol[k];
                         T sum = T(0);
                         auto add = [&](T cf, T v) {
                             if constexpr (sizeof(T) == 8) sum = sum + cf * v;
                             else sum = fmaf(cf, v, sum);
                         };
+                        // the plane carries a zero halo of width O: no tests, fixed offsets
 #pragma unroll
                         for (int l = O; l >= 1; l--) add(wgt(2, l), win[k][O - l]);
 #pragma unroll
-                        for (int l = O; l >= 1; l--) add(wgt(1, l), cj[k] - l >= 0 ? pl[c - l * nx] : T(0));
+                        for (int l = O; l >= 1; l--) add(wgt(1, l), pc[-l * px]);
 #pragma unroll
-                        for (int l = O; l >= 1; l--) add(wgt(0, l), ci[k] - l >= 0 ? pl[c - l] : T(0));
+                        for (int l = O; l >= 1; l--) add(wgt(0, l), pc[-l]);
                         add(dg[k], win[k][O]);
 #pragma unroll
-                        for (int l = 1; l <= O; l++) add(wgt(0, l), ci[k] + l < nx ? pl[c + l] : T(0));
+                        for (int l = 1; l <= O; l++) add(wgt(0, l), pc[l]);
 #pragma unroll
-                        for (int l = 1; l <= O; l++) add(wgt(1, l), cj[k] + l < ny ? pl[c + l * nx] : T(0));
+                        for (int l = 1; l <= O; l++) add(wgt(1, l), pc[l * px]);
 #pragma unroll
                         for (int l = 1; l <= O; l++) add(wgt(2, l), win[k][O + l]);
-                        Bu[c + z * ncol] = sum;
+                        Bu[col[k] + z * ncol] = sum;
                     }
 #pragma unroll
                     for (int q = 0; q < 2 * O; q++) win[k][q] = win[k][q + 1];
@@ -356,6 +389,7 @@ __global__ void __launch_bounds__(512) lanczos_zmarch(KParams P)
                         const int nzr = zhi - zlo + 1;
                         if constexpr (LAYOUT == 1) {
                             using V2 = typename std::conditional<sizeof(T) == 8, double2, float2>::type;
+                            const SdColumn<T> pz = sd_column(x, y);
                             const V2 *r2 = reinterpret_cast<const V2 *>(rad + rb);
                             const int zs2 = dm.x * dm.y;
                             int i = 0;
@@ -371,9 +405,8 @@ __global__ void __launch_bounds__(512) lanczos_zmarch(KParams P)
 #pragma unroll
                                 for (int u = 0; u < U; u++) {
                                     const T zc = (T)(P.hgrid[2] * (cube_lo[2] + zlo + i + u) - P.range[2] - cen.z);
-                                    const T tt[2] = {t4[u].x, t4[u].y};
                                     T cv[NLM];
-                                    chi_channels<T, NLM, 1>(P, type, nlm, tt, x, y, zc, cv);
+                                    sd_channels<T, NLM>(pz, t4[u].x, t4[u].y, zc, cv);
                                     T sum = s4[u];
 #pragma unroll
                                     for (int ch = 0; ch < NLM; ch++) sum = fma(cv[ch], al[ch], sum);
@@ -384,9 +417,8 @@ __global__ void __launch_bounds__(512) lanczos_zmarch(KParams P)
                             for (; i < nzr; i++) {
                                 const V2 t2 = r2[i * zs2];
                                 const T zc = (T)(P.hgrid[2] * (cube_lo[2] + zlo + i) - P.range[2] - cen.z);
-                                const T tt[2] = {t2.x, t2.y};
                                 T cv[NLM];
-                                chi_channels<T, NLM, 1>(P, type, nlm, tt, x, y, zc, cv);
+                                sd_channels<T, NLM>(pz, t2.x, t2.y, zc, cv);
                                 T sum = bu[i * ncol];
 #pragma unroll
                                 for (int ch = 0; ch < NLM; ch++) sum = fma(cv[ch], al[ch], sum);
