@@ -127,6 +127,14 @@ int sq_last_timing(sq_ctx *ctx, double *lanczos_ms, double *ql_ms);
  * lsdft_matvec.cc:25-293).  x, y: host vectors of nc entries. */
 int sq_apply_h(sq_ctx *ctx, int nonlocal_mode, int Kp, int Jp, int Ip, const double *x, double *y);
 
+/* The nonlocal part alone, accumulated: y += V_nl x on node (Kp,Jp,Ip), each
+ * point's projector terms added onto its incoming y in ascending image and
+ * channel order.  Replaces Mult_NonlocalVector (nonlocal_mode
+ * SQ_NL_REF_COLLAPSED, lsdft_matvec.cc:25-149) and Mult_HamiltonianVector
+ * (SQ_NL_KB, lsdft_matvec.cc:155-293), which add into V2.  x, y: host
+ * vectors of nc entries. */
+int sq_apply_nonlocal(sq_ctx *ctx, int nonlocal_mode, int Kp, int Jp, int Ip, const double *x, double *y);
+
 /* Batched QL nodes and weights on the device for host tridiagonals:
  * diag[count][n], subdiag[count][n] (subdiag[.][n-1] ignored), outputs
  * lambda/w [count][n], lmin/lmax [count].  TridiagEigenVecSolve_NodesAndWeights,

@@ -62,6 +62,7 @@ def lib():
         L.sqo_info.argtypes = [vp, C.POINTER(Info)]
         L.sqo_info_size.restype = C.c_int
         L.sqo_apply_h.argtypes = [vp, C.c_int, dp, C.c_double, C.c_int, C.c_int, C.c_int, dp, dp]
+        L.sqo_apply_nonlocal.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, dp, dp]
         L.sqo_lanczos.argtypes = [vp, C.c_int, dp, C.c_double, C.c_int, C.c_int, C.c_int, dp, dp]
         L.sqo_tridiag_nodes_weights.argtypes = [dp, dp, C.c_int, dp, dp, dp, dp]
         L.sqo_nodes_and_weights.argtypes = [vp, C.c_int, dp, C.c_double, ip, C.c_int, dp, dp, dp, dp, C.c_int]
@@ -250,6 +251,14 @@ class System:
         if veff is None or np.isscalar(veff):
             return None, (0.333 if veff is None else float(veff))
         return np.ascontiguousarray(veff, np.float64), 0.0
+
+    def apply_nonlocal(self, node, x, y, mode=MODE_REF_COLLAPSED):
+        """y + V_nl x (Mult_NonlocalVector / Mult_HamiltonianVector accumulate into V2)."""
+        K, J, I_ = node
+        x = np.ascontiguousarray(x, np.float64)
+        y = np.array(y, np.float64, copy=True)
+        _check(lib().sqo_apply_nonlocal(self._h, mode, K, J, I_, _dp(x), _dp(y)), "apply_nonlocal")
+        return y
 
     def apply_h(self, node, x, mode=MODE_REF_COLLAPSED, veff=None):
         K, J, I_ = node

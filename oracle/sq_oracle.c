@@ -980,6 +980,18 @@ static void nonlocal_apply(const sqo_sys *S, int mode, double *alpha, int Kp, in
     }
 }
 
+/* The nonlocal part alone, accumulated into y: Mult_NonlocalVector (mode 0)
+ * or Mult_HamiltonianVector (mode 1) as called at lsdft_scf.cc:123,140. */
+int sqo_apply_nonlocal(sqo_sys *S, int mode, int Kp, int Jp, int Ip, const double *x, double *y)
+{
+    if (!S->have_nl || (mode != 0 && mode != 1)) return SQO_ERR_ARG;
+    double *alpha = (double *)calloc(S->IP_length > 0 ? S->IP_length : 1, sizeof(double));
+    if (!alpha) return SQO_ERR_ALLOC;
+    nonlocal_apply(S, mode, alpha, Kp, Jp, Ip, x, y);
+    free(alpha);
+    return SQO_OK;
+}
+
 /* One H application on node (Kp,Jp,Ip): y = (L + diag V_eff) x + V_nl x. */
 int sqo_apply_h(sqo_sys *S, int mode, const double *veff_field, double veff_const, int Kp, int Jp, int Ip,
                 const double *x, double *y)

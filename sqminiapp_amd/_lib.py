@@ -24,7 +24,7 @@ STATUS = {0: "SQ_OK", 1: "SQ_ERR_IO", 2: "SQ_ERR_FORMAT", 3: "SQ_ERR_RANGE", 4: 
 # every symbol include/sq_amd.h declares
 EXPORTS = ["sq_last_error", "sq_default_options", "sq_create_from_deck", "sq_destroy", "sq_get_info", "sq_set_veff",
            "sq_run", "sq_get_nodes_and_weights", "sq_get_lanczos", "sq_nodes_and_weights", "sq_last_timing",
-           "sq_apply_h", "sq_tridiag_nodes_weights", "sq_density_fermi_energy", "sq_synchronize", "sq_get_image"]
+           "sq_apply_h", "sq_apply_nonlocal", "sq_tridiag_nodes_weights", "sq_density_fermi_energy", "sq_synchronize", "sq_get_image"]
 
 
 class SQError(RuntimeError):
@@ -74,6 +74,7 @@ def lib():
     L.sq_nodes_and_weights.argtypes = [vp, C.c_int, C.c_int, lg, lg, dp, dp, dp, dp]
     L.sq_last_timing.argtypes = [vp, dp, dp]
     L.sq_apply_h.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, dp, dp]
+    L.sq_apply_nonlocal.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, dp, dp]
     L.sq_tridiag_nodes_weights.argtypes = [vp, dp, dp, C.c_int, lg, dp, dp, dp, dp]
     L.sq_density_fermi_energy.argtypes = [vp, C.c_double, C.c_double, dp, dp, dp, dp]
     L.sq_synchronize.argtypes = [vp]
@@ -192,6 +193,16 @@ class Context:
             raise ValueError("vector size differs from the cube")
         y = np.zeros_like(x)
         _check(lib().sq_apply_h(self._h, int(mode), int(K), int(J), int(I_), _dp(x), _dp(y)))
+        return y
+
+    def apply_nonlocal(self, node, x, y, mode=SQ_NL_REF_COLLAPSED):
+        """Returns y + V_nl x, the accumulation of Mult_NonlocalVector / Mult_HamiltonianVector."""
+        K, J, I_ = node
+        x = np.ascontiguousarray(x, np.float64)
+        y = np.array(y, np.float64, copy=True)
+        if x.size != self.info.nc or y.size != self.info.nc:
+            raise ValueError("vector size differs from the cube")
+        _check(lib().sq_apply_nonlocal(self._h, int(mode), int(K), int(J), int(I_), _dp(x), _dp(y)))
         return y
 
     def tridiag_nodes_weights(self, diag, subdiag):

@@ -36,6 +36,11 @@ int set_err(int code, const std::string &msg)
 constexpr int kRedBlocks = 512;
 }  // namespace
 
+namespace sqamd {
+// Error entry for the reference-named host layer (sq_lsdft.cpp): same message slot.
+int record_error(int code, const std::string &msg) { return set_err(code, msg); }
+}  // namespace sqamd
+
 struct sq_ctx {
     System sys;
     int device = 0, n_cu = 0, grid = 0, img_cap = 0, nlm_tmpl = 6;
@@ -621,7 +626,7 @@ int sq_last_timing(sq_ctx *c, double *lanczos_ms, double *ql_ms)
     return SQ_OK;
 }
 
-int sq_apply_h(sq_ctx *c, int mode, int Kp, int Jp, int Ip, const double *x, double *y)
+static int apply_common(sq_ctx *c, int mode, int Kp, int Jp, int Ip, const double *x, double *y, int nonlocal_only)
 {
     if (!c || !x || !y) return set_err(SQ_ERR_ARG, "null argument");
     if (!c->stream) return set_err(SQ_ERR_STATE, "host-only context (created with device < 0)");
@@ -641,19 +646,31 @@ int sq_apply_h(sq_ctx *c, int mode, int Kp, int Jp, int Ip, const double *x, dou
     KParams P = make_params(c, mode, SQ_FP64, id, 1);
     int zero = 0;
     if (hipMemcpy(dx, x, nc * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+        (nonlocal_only && hipMemcpy(dy, y, nc * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) ||
         hipMemcpy(c->status, &zero, sizeof(int), hipMemcpyHostToDevice) != hipSuccess ||
-        launch_apply(P, id, dx, dy, c->lds, c->stream) != hipSuccess || hipStreamSynchronize(c->stream) != hipSuccess ||
+        launch_apply(P, id, dx, dy, nonlocal_only, c->lds, c->stream) != hipSuccess ||
+        hipStreamSynchronize(c->stream) != hipSuccess ||
         hipMemcpy(y, dy, nc * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
-        rc = set_err(SQ_ERR_DEVICE, "apply_h device step failed");
+        rc = set_err(SQ_ERR_DEVICE, "apply device step failed");
     if (!rc) {
         int st = 0;
         (void)hipMemcpy(&st, c->status, sizeof(int), hipMemcpyDeviceToHost);
-        if (st) rc = set_err(SQ_ERR_ARG, "apply_h: image list overflow");
+        if (st) rc = set_err(SQ_ERR_ARG, "apply: image list overflow");
     }
     (void)hipFree(dx);
     (void)hipFree(dy);
     c->have_run = false;  // status slot 0 was reused
     return rc;
+}
+
+int sq_apply_h(sq_ctx *c, int mode, int Kp, int Jp, int Ip, const double *x, double *y)
+{
+    return apply_common(c, mode, Kp, Jp, Ip, x, y, 0);
+}
+
+int sq_apply_nonlocal(sq_ctx *c, int mode, int Kp, int Jp, int Ip, const double *x, double *y)
+{
+    return apply_common(c, mode, Kp, Jp, Ip, x, y, 1);
 }
 
 int sq_tridiag_nodes_weights(sq_ctx *c, const double *diag, const double *subdiag, int n, long count, double *lambda,

@@ -195,23 +195,25 @@ __device__ void images_alpha(const KParams &P, const int *img_list, int nimg, co
 template <typename T, int NLM>
 __device__ __forceinline__ T apply_point(const KParams &P, int p, int li, int lj, int lk, const int cube_lo[3],
                                          const T *vraw, T vs, const T *diag_ext, const T *chi, const short *local_of,
-                                         const double *alpha, int64_t E)
+                                         const double *alpha, int64_t E, bool with_local = true, T init = T(0))
 {
     const int nx = P.nloc[0], ny = P.nloc[1], nz = P.nloc[2], nxy = nx * ny, o = P.order;
-    T sum = T(0);
+    T sum = init;
     auto term = [&](T cf, int q) {
         if constexpr (sizeof(T) == 8) sum = sum + cf * (vs * vraw[q]);
         else sum = fmaf(cf, vs * vraw[q], sum);
     };
     const T *cx = reinterpret_cast<const T *>(P.cx_ptr), *cy = reinterpret_cast<const T *>(P.cy_ptr),
             *cz = reinterpret_cast<const T *>(P.cz_ptr);
-    for (int l = o; l >= 1; l--) if (lk - l >= 0) term(cz[l], p - l * nxy);
-    for (int l = o; l >= 1; l--) if (lj - l >= 0) term(cy[l], p - l * nx);
-    for (int l = o; l >= 1; l--) if (li - l >= 0) term(cx[l], p - l);
-    term(diag_ext[E], p);
-    for (int l = 1; l <= o; l++) if (li + l < nx) term(cx[l], p + l);
-    for (int l = 1; l <= o; l++) if (lj + l < ny) term(cy[l], p + l * nx);
-    for (int l = 1; l <= o; l++) if (lk + l < nz) term(cz[l], p + l * nxy);
+    if (with_local) {
+        for (int l = o; l >= 1; l--) if (lk - l >= 0) term(cz[l], p - l * nxy);
+        for (int l = o; l >= 1; l--) if (lj - l >= 0) term(cy[l], p - l * nx);
+        for (int l = o; l >= 1; l--) if (li - l >= 0) term(cx[l], p - l);
+        term(diag_ext[E], p);
+        for (int l = 1; l <= o; l++) if (li + l < nx) term(cx[l], p + l);
+        for (int l = 1; l <= o; l++) if (lj + l < ny) term(cy[l], p + l * nx);
+        for (int l = 1; l <= o; l++) if (lk + l < nz) term(cz[l], p + l * nxy);
+    }
     // nonlocal scatter, images in ascending order, channels ascending
     const int e0 = P.cov_off[E], e1 = P.cov_off[E + 1];
     const int gx = cube_lo[0] + li, gy = cube_lo[1] + lj, gz = cube_lo[2] + lk;

@@ -54,7 +54,8 @@ hipError_t launch_lanczos(const KParams &P, int precision, int grid, size_t lds,
 hipError_t launch_zmarch(const KParams &P, int precision, int grid, size_t lds, hipStream_t st);
 int zmarch_workgroup(int ncol);
 int zmarch_occupancy(const KParams &P, int precision, size_t lds);
-hipError_t launch_apply(const KParams &P, long id, const double *x, double *y, size_t lds, hipStream_t st);
+hipError_t launch_apply(const KParams &P, long id, const double *x, double *y, int nonlocal_only, size_t lds,
+                        hipStream_t st);
 hipError_t launch_ql(const double *a, const double *b, int n, int stride, long count, double *lam, double *w,
                      double *lmin, double *lmax, int *status, double *ws, hipStream_t st);
 hipError_t launch_occupation(const double *lam, const double *w, long nq, double beta, double lf, double *partial,

@@ -154,9 +154,12 @@ __global__ void __launch_bounds__(kWG) lanczos_kernel(KParams P)
     }
 }
 
-// One H application y = H x on one node (device vectors x, y of nc entries).
+// One H application on one node (device vectors x, y of nc entries):
+// y = H x, or with nonlocal_only y += V_nl x, each point's projector terms
+// added onto its incoming y in ascending image order as Mult_NonlocalVector
+// accumulates into V2 (lsdft_matvec.cc:110-142).
 template <typename T, int NLM>
-__global__ void __launch_bounds__(kWG) apply_kernel(KParams P, long id, const T *x, T *y)
+__global__ void __launch_bounds__(kWG) apply_kernel(KParams P, long id, const T *x, T *y, int nonlocal_only)
 {
     extern __shared__ __align__(16) unsigned char smem[];
     int *img_list = reinterpret_cast<int *>(smem);
@@ -196,7 +199,8 @@ __global__ void __launch_bounds__(kWG) apply_kernel(KParams P, long id, const T 
         const int64_t E = ((int64_t)(cube_lo[2] + lk - P.ext0[2]) * P.extn[1] + (cube_lo[1] + lj - P.ext0[1])) * P.extn[0] +
                           (cube_lo[0] + li - P.ext0[0]);
         y[p] = apply_point<T, NLM>(P, p, li, lj, lk, cube_lo, x, T(1), reinterpret_cast<const T *>(P.diag_ext),
-                                   reinterpret_cast<const T *>(P.chi), local_of, alpha, E);
+                                   reinterpret_cast<const T *>(P.chi), local_of, alpha, E, !nonlocal_only,
+                                   nonlocal_only ? y[p] : T(0));
     }
 }
 
@@ -392,11 +396,14 @@ hipError_t launch_lanczos(const KParams &P, int precision, int grid, size_t lds,
     return hipErrorInvalidValue;
 }
 
-hipError_t launch_apply(const KParams &P, long id, const double *x, double *y, size_t lds, hipStream_t st)
+hipError_t launch_apply(const KParams &P, long id, const double *x, double *y, int nonlocal_only, size_t lds,
+                        hipStream_t st)
 {
-    if (P.nlm_max <= 6) hipLaunchKernelGGL((apply_kernel<double, 6>), dim3(1), dim3(kWG), lds, st, P, id, x, y);
+    if (P.nlm_max <= 6)
+        hipLaunchKernelGGL((apply_kernel<double, 6>), dim3(1), dim3(kWG), lds, st, P, id, x, y, nonlocal_only);
     else if (P.nlm_max <= kMaxChannels)
-        hipLaunchKernelGGL((apply_kernel<double, kMaxChannels>), dim3(1), dim3(kWG), lds, st, P, id, x, y);
+        hipLaunchKernelGGL((apply_kernel<double, kMaxChannels>), dim3(1), dim3(kWG), lds, st, P, id, x, y,
+                           nonlocal_only);
     else return hipErrorInvalidValue;
     return hipGetLastError();
 }

@@ -1,0 +1,128 @@
+// sq_main.cpp -- command-line driver with the steps of the reference's main
+// (src/main.cc:17-75) over the reference-named host layer:
+//
+//   sq_main -name <dir>/<name> [-mode collapsed|kb] [-fp32] [-phantom [x y z]]
+//           [-rank_box xcor ycor zcor lx ly lz] [-rank r] [-nqp n] [-device d]
+//           [-out <DenFilename>] [-density] [-verify_files]
+//
+// It prints the reference's progress lines and its two SQ timers, then writes
+// <DenFilename>_<rank>.lambdas (Print_MinMaxLambda) and, with -density, the
+// Fermi level, energies and <DenFilename>_<rank>.den (Print_ElecDens).  Any
+// failing step prints sq_last_error() and exits with status 1, where the
+// reference exits too.  -verify_files reads the written files back with
+// Read_MinMaxLambda / Read_ElecDens and checks them against the results.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <ctime>
+#include <string>
+
+#include "../../include/sq_lsdft.h"
+
+static void die(const char *step, int rc)
+{
+    fprintf(stderr, "%s failed (status %d): %s\n", step, rc, sq_last_error());
+    exit(1);
+}
+
+static double now()
+{
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+int main(int argc, char **argv)
+{
+    std::string path;
+    SQ_LSDFT lsdft;
+    sq_options opt;
+    sq_default_options(&opt);
+    bool density = false, verify = false;
+    std::string out;
+    for (int a = 1; a < argc; a++) {
+        const std::string s = argv[a];
+        auto need = [&](int n) {
+            if (a + n >= argc) { fprintf(stderr, "%s needs %d value(s)\n", s.c_str(), n); exit(1); }
+        };
+        if (s == "-name") { need(1); path = argv[++a]; }
+        else if (s == "-mode") {
+            need(1);
+            const std::string m = argv[++a];
+            if (m == "collapsed") lsdft.nonlocal_mode = SQ_NL_REF_COLLAPSED;
+            else if (m == "kb") lsdft.nonlocal_mode = SQ_NL_KB;
+            else { fprintf(stderr, "-mode is collapsed or kb\n"); return 1; }
+        }
+        else if (s == "-fp32") lsdft.precision = SQ_FP32;
+        else if (s == "-phantom") {
+            opt.phantom_atom = 1;
+            auto is_num = [](const char *t) { char *e; strtod(t, &e); return e != t && *e == 0; };
+            if (a + 3 < argc && is_num(argv[a + 1]) && is_num(argv[a + 2]) && is_num(argv[a + 3])) {
+                for (int d = 0; d < 3; d++) opt.phantom_pos[d] = atof(argv[++a]);
+            }
+        }
+        else if (s == "-rank_box") { need(6); for (int d = 0; d < 6; d++) opt.rank_box[d] = atoi(argv[++a]); }
+        else if (s == "-rank") { need(1); lsdft.rank = atoi(argv[++a]); }
+        else if (s == "-nqp") { need(1); opt.nqp_override = atoi(argv[++a]); }
+        else if (s == "-device") { need(1); opt.device = atoi(argv[++a]); }
+        else if (s == "-out") { need(1); out = argv[++a]; }
+        else if (s == "-density") density = true;
+        else if (s == "-verify_files") verify = true;
+        else { fprintf(stderr, "unknown option %s\n", s.c_str()); return 1; }
+    }
+    if (path.empty()) { fprintf(stderr, "usage: sq_main -name <dir>/<name> [options]\n"); return 1; }
+    const size_t slash = path.find_last_of('/');
+    const std::string dir = slash == std::string::npos ? "." : path.substr(0, slash);
+    const std::string name = slash == std::string::npos ? path : path.substr(slash + 1);
+    lsdft.DenFilename = out.empty() ? "./" + name : out;
+
+    const clock_t begin0 = clock();
+    const double w1 = now();
+    int rc = Initialize(&lsdft, dir.c_str(), name.c_str(), &opt);
+    if (rc) die("Initialize", rc);
+    printf("done influencing atoms local\n");
+    printf("done influencing atoms nonlocal\n");
+    if ((rc = (int)CalculateNonlocalProjectors(&lsdft))) die("CalculateNonlocalProjectors", rc);
+    printf("Nonlocal projectors calculated\n");
+    if ((rc = NonlocalProjectors(&lsdft))) die("NonlocalProjectors", rc);
+    printf("rank=%d,NEW Nonlocal projectors calculated\n", lsdft.rank);
+    printf("rank=%d, number of FD points=%d \n", lsdft.rank, lsdft.lxdim * lsdft.lydim * lsdft.lzdim);
+
+    const clock_t begin = clock();
+    const double v1 = now();
+    if ((rc = (int)NodesAndWeights_LSSGQ(&lsdft))) die("NodesAndWeights_LSSGQ", rc);
+    const double v2 = now();
+    const clock_t end = clock();
+    printf("rank=%d, Time spent in SQ=%lf \n", lsdft.rank, (double)(end - begin) / CLOCKS_PER_SEC);
+    printf("rank=%d, Petsc Time spent in SQ=%lf \n", lsdft.rank, v2 - v1);
+    printf("Done LSSGQ \n");
+    if ((rc = Print_MinMaxLambda(&lsdft))) die("Print_MinMaxLambda", rc);
+    if (density) {
+        if ((rc = CalculateDensity_lssgq(&lsdft))) die("CalculateDensity_lssgq", rc);
+        SystemEnergy_Calc_lssgq(&lsdft);
+        if ((rc = Print_ElecDens(&lsdft))) die("Print_ElecDens", rc);
+    }
+    if (verify) {
+        // %0.16lf keeps 16 decimals: values of magnitude < 100 come back within 1e-14
+        const auto lmin = lsdft.lambda_min, lmax = lsdft.lambda_max, rho = lsdft.elecDensRho;
+        lsdft.lambda_min.assign(lmin.size(), 0.0);
+        lsdft.lambda_max.assign(lmax.size(), 0.0);
+        lsdft.elecDensRho.assign(rho.size(), 0.0);
+        if ((rc = Read_MinMaxLambda(&lsdft))) die("Read_MinMaxLambda", rc);
+        if (density && (rc = Read_ElecDens(&lsdft))) die("Read_ElecDens", rc);
+        double err = 0;
+        for (size_t q = 0; q < lmin.size(); q++) {
+            err = std::max(err, std::fabs(lmin[q] - lsdft.lambda_min[q]));
+            err = std::max(err, std::fabs(lmax[q] - lsdft.lambda_max[q]));
+            if (density) err = std::max(err, std::fabs(rho[q] - lsdft.elecDensRho[q]));
+        }
+        if (!(err <= 1e-14)) { fprintf(stderr, "file round trip differs by %.3e\n", err); return 1; }
+        printf("files read back: max difference %.3e\n", err);
+    }
+    const double w2 = now();
+    printf("rank=%d, Total time spent=%lf \n", lsdft.rank, (double)(clock() - begin0) / CLOCKS_PER_SEC);
+    printf("rank=%d, Petsc Total time spent=%lf \n", lsdft.rank, w2 - w1);
+    Objects_Destroy(&lsdft);
+    return 0;
+}

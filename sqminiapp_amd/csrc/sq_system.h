@@ -1,6 +1,6 @@
 // sq_system.h -- host-side model of one SQ problem (deck, pseudopotentials,
 // projector images, coverage map), built without PETSc by sq_setup.cpp and
-// uploaded to the device by sq_device.hip.
+// uploaded to the device by sq_api.cpp.
 #pragma once
 #include <cstddef>
 #include <cstdint>

@@ -61,6 +61,26 @@ def test_apply_h_against_dense_projectors(oracle_mod, tiny_deck, mode):
         assert np.allclose(H, H.T, rtol=0, atol=1e-9 * np.abs(H).max())
 
 
+@pytest.mark.parametrize("mode", [0, 1])
+def test_nonlocal_part_accumulates_onto_v2(oracle_mod, tiny_deck, mode):
+    """Mult_NonlocalVector / Mult_HamiltonianVector add V_nl x onto what V2 holds."""
+    S = oracle_mod.System(*tiny_deck)
+    S.set_phantom(True)
+    S.build_nonlocal()
+    node = (8, 9, 5)
+    A, chans, gam = _dense_parts(oracle_mod, S, node)
+    if mode == 1:
+        V = np.einsum("c,pci,pcj->ij", gam, chans, chans)
+    else:
+        V = np.einsum("pi,pj->ij", chans.sum(1), np.einsum("c,pci->pi", gam, chans))
+    rng = np.random.default_rng(8)
+    x, y0 = rng.standard_normal((2, A.shape[0]))
+    y = S.apply_nonlocal(node, x, y0, mode=mode)
+    assert rel_err(y - y0, V @ x) < 1e-12
+    # with V2 = (L + V_eff) x first, the sum is the full H application
+    assert rel_err(S.apply_nonlocal(node, x, A @ x + 0.333 * x, mode=mode), S.apply_h(node, x, mode=mode)) < 1e-14
+
+
 def test_lanczos_full_spectrum_on_tiny_cube(oracle_mod, tmp_path):
     """Against the dense eigendecomposition H = V diag(ev) V^T of a 5^3 cube:
     the k-node Gauss rule integrates ev^m against the weights V[c]^2 exactly
