@@ -105,12 +105,6 @@ __global__ void __launch_bounds__(512) lanczos_zmarch(KParams P)
         else s_w[tid / 7][tid % 7] = P.cw32[tid / 7][tid % 7];
     }
     __syncthreads();
-    T wreg[3][O + 1];
-#pragma unroll
-    for (int d = 0; d < 3; d++)
-#pragma unroll
-        for (int l = 0; l <= O; l++) wreg[d][l] = s_w[d][l];
-    auto wgt = [&](int d, int l) -> T { return wreg[d][l]; };
 
     // Nodes are taken in passes of gridDim.x; within a pass the workgroups of
     // one XCD (blockIdx % 8 under round-robin dispatch) take a contiguous run,
@@ -304,62 +298,96 @@ __global__ void __launch_bounds__(512) lanczos_zmarch(KParams P)
             __syncthreads();
             images_alpha<NLM>(P, img_list, nimg, A, alpha);
             __syncthreads();
-            // ---- apply, marching along z
-            T win[CPT][2 * O + 1];
+            // ---- nonlocal part of <v, H v> from the gathered sums: the scatter adds
+            // sum_ch chi_ch alpha_ch per image, so <v, V_nl v> = sum alpha . A
+            double pn = 0.0;
+            for (int e = tid; e < nimg * NLM; e += WG) {
+                const int li = e / NLM, ch = e - li * NLM;
+                if (ch < (tdim[li].w & 255)) pn += alpha[e] * A[e];
+            }
+            // ---- local part, marching along z: u = (L + diag) v into Bu and <v, u>.
+            // The z loop is unrolled by the window length so the window is a ring
+            // with fixed register slots: plane z + d sits in slot (z + d + O) % W.
+            constexpr int W = 2 * O + 1;
+            // stencil weights in vector registers for the march only (read after
+            // the barriers above, so they are not held through the projector loops)
+            T wreg[3][O + 1];
+#pragma unroll
+            for (int d = 0; d < 3; d++)
+#pragma unroll
+                for (int l = 1; l <= O; l++) wreg[d][l] = s_w[d][l];
+            auto wgt = [&](int d, int l) -> T { return wreg[d][l]; };
+            double pa = 0.0;
+            CSum ca{0.f, 0.f};
+            T win[CPT][W];
 #pragma unroll
             for (int k = 0; k < CPT; k++)
 #pragma unroll
-                for (int q = 0; q <= 2 * O; q++) {
-                    const int z = q - O;
+                for (int q = 0; q < W; q++) {
+                    const int z = q - O;   // slot q holds plane q - O before the first step
                     win[k][q] = (has[k] && z >= 0 && z < nz) ? sv * Bv[col[k] + z * ncol] : T(0);
                 }
-            for (int z = 0; z < nz; z++) {
-                T *pl = plane + (z & 1) * pplane;
-                T next[CPT], dg[CPT];
+            T dgn[CPT];
 #pragma unroll
-                for (int k = 0; k < CPT; k++) {
-                    next[k] = T(0);
-                    dg[k] = T(0);
-                    if (has[k]) {
-                        pl[pcol[k]] = win[k][O];
-                        if (z + O + 1 < nz) next[k] = sv * Bv[col[k] + (z + O + 1) * ncol];
-                        dg[k] = diag_ext[ecol[k] + z * eplane];
+            for (int k = 0; k < CPT; k++) dgn[k] = has[k] ? diag_ext[ecol[k]] : T(0);
+            for (int zb = 0; zb < nz; zb += W) {
+#pragma unroll
+                for (int s = 0; s < W; s++) {
+                    const int z = zb + s;
+                    if (z < nz) {   // uniform: the last block is partial
+                    T *pl = plane + (z & 1) * pplane;
+                    T next[CPT], dg[CPT];
+#pragma unroll
+                    for (int k = 0; k < CPT; k++) {
+                        next[k] = T(0);
+                        dg[k] = dgn[k];
+                        if (has[k]) {
+                            pl[pcol[k]] = win[k][(s + O) % W];
+                            if (z + O + 1 < nz) next[k] = sv * Bv[col[k] + (z + O + 1) * ncol];
+                            if (z + 1 < nz) dgn[k] = diag_ext[ecol[k] + (z + 1) * eplane];
+                        }
                     }
-                }
-                __syncthreads();
+                    __syncthreads();
 #pragma unroll
-                for (int k = 0; k < CPT; k++) {
-                    if (has[k]) {
-                        const T *pc = pl + pcol[k];
-                        T sum = T(0);
-                        auto add = [&](T cf, T v) {
-                            if constexpr (sizeof(T) == 8) sum = sum + cf * v;
-                            else sum = fmaf(cf, v, sum);
-                        };
-                        // the plane carries a zero halo of width O: no tests, fixed offsets
+                    for (int k = 0; k < CPT; k++) {
+                        if (has[k]) {
+                            const T *pc = pl + pcol[k];
+                            // three independent fma chains; the plane's zero halo
+                            // supplies the out-of-cube neighbours
+                            T sx = T(0), sy = T(0), sz = T(0);
 #pragma unroll
-                        for (int l = O; l >= 1; l--) add(wgt(2, l), win[k][O - l]);
+                            for (int l = 1; l <= O; l++) {
+                                sz = fma(wgt(2, l), win[k][(s + O - l) % W], sz);
+                                sy = fma(wgt(1, l), pc[-l * px], sy);
+                                sx = fma(wgt(0, l), pc[-l], sx);
+                            }
 #pragma unroll
-                        for (int l = O; l >= 1; l--) add(wgt(1, l), pc[-l * px]);
-#pragma unroll
-                        for (int l = O; l >= 1; l--) add(wgt(0, l), pc[-l]);
-                        add(dg[k], win[k][O]);
-#pragma unroll
-                        for (int l = 1; l <= O; l++) add(wgt(0, l), pc[l]);
-#pragma unroll
-                        for (int l = 1; l <= O; l++) add(wgt(1, l), pc[l * px]);
-#pragma unroll
-                        for (int l = 1; l <= O; l++) add(wgt(2, l), win[k][O + l]);
-                        Bu[col[k] + z * ncol] = sum;
+                            for (int l = 1; l <= O; l++) {
+                                sz = fma(wgt(2, l), win[k][(s + O + l) % W], sz);
+                                sy = fma(wgt(1, l), pc[l * px], sy);
+                                sx = fma(wgt(0, l), pc[l], sx);
+                            }
+                            const T vc = win[k][(s + O) % W];
+                            const T u = fma(dg[k], vc, sx) + (sy + sz);
+                            Bu[col[k] + z * ncol] = u;
+                            if constexpr (sizeof(T) == 8) pa = fma(vc, u, pa);
+                            else ca = csum_add(ca, vc * u);
+                        }
+                        // plane z - O leaves the window; plane z + O + 1 takes its slot
+                        win[k][s % W] = next[k];
                     }
-#pragma unroll
-                    for (int q = 0; q < 2 * O; q++) win[k][q] = win[k][q + 1];
-                    win[k][2 * O] = next[k];
+                    }
                 }
             }
-            // ---- nonlocal: chi*alpha of each column's images, ascending, then <v,u>
-            double pa = 0.0;
-            CSum ca{0.f, 0.f};
+            double a;
+            if constexpr (sizeof(T) == 8) a = block_sum<NW>(pa + pn, red);
+            else a = block_csum<NW>(ca, red) + block_sum<NW>(pn, red);
+            // ---- per column: nonlocal chi*alpha of the column's images, ascending,
+            // added onto the local part, then w = (u - a v) - bprev vprev over
+            // vprev's storage and ||w||^2
+            const T ta = T(-a), tb = T(-bprev);
+            double pb = 0.0;
+            CSum cb{0.f, 0.f};
 #pragma unroll
             for (int k = 0; k < CPT; k++) {
                 if (!has[k]) continue;
@@ -445,32 +473,15 @@ __global__ void __launch_bounds__(512) lanczos_zmarch(KParams P)
                     }
                 }
                 for (int z = 0; z < nz; z++) {
-                    const T vc = sv * Bv[c + z * ncol];
-                    const T u = Bu[c + z * ncol];
-                    if constexpr (sizeof(T) == 8) pa += vc * u;
-                    else ca = csum_add(ca, vc * u);
-                }
-            }
-            double a;
-            if constexpr (sizeof(T) == 8) a = block_sum<NW>(pa, red);
-            else a = block_csum<NW>(ca, red);
-            // ---- w = (u - a v) - bprev vprev over vprev's storage
-            const T ta = T(-a), tb = T(-bprev);
-            double pb = 0.0;
-            CSum cb{0.f, 0.f};
-#pragma unroll
-            for (int k = 0; k < CPT; k++) {
-                if (!has[k]) continue;
-                for (int z = 0; z < nz; z++) {
-                    const int p = col[k] + z * ncol;
+                    const int p = c + z * ncol;
                     const T vc = sv * Bv[p];
                     const T vpc = svp * Bvp[p];
                     const T u = Bu[p];
                     T w;
                     if constexpr (sizeof(T) == 8) {
-                        w = u + ta * vc;
-                        w = w + tb * vpc;
-                        pb += w * w;
+                        w = fma(ta, vc, u);
+                        w = fma(tb, vpc, w);
+                        pb = fma(w, w, pb);
                     } else {
                         w = fmaf(ta, vc, u);
                         w = fmaf(tb, vpc, w);
