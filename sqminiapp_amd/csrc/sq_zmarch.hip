@@ -74,6 +74,7 @@ __global__ void __launch_bounds__(512) lanczos_zmarch(KParams P)
     T *plane = reinterpret_cast<T *>(smem + P.lds_plane);
     __shared__ int s_count;
     __shared__ T s_w[3][8];
+    const T hzt = (T)P.hgrid[2];   // plane u of a trip sits at z0 + u h_z
 
     const int nx = P.nloc[0], ny = P.nloc[1], nz = P.nloc[2];
     const int ncol = nx * ny, nc = P.nc;
@@ -229,41 +230,45 @@ __global__ void __launch_bounds__(512) lanczos_zmarch(KParams P)
                     const T y = (T)(P.hgrid[1] * gy - P.range[1] - cen.y);
                     const T *vb = Bv + p0;
                     if constexpr (LAYOUT == 1) {
-                        // two radial factors per point, one 16-byte load
+                        // Two radial factors per point, one 16-byte load.  Along the
+                        // column the d channels are t2 * (c0 + c1 z + c2 z^2) with
+                        // column constants, so the z loop keeps four sums and the
+                        // six channels are formed once per column.
                         using V2 = typename std::conditional<sizeof(T) == 8, double2, float2>::type;
-                        const SdColumn<T> pz = sd_column(x, y);
                         const V2 *r2 = reinterpret_cast<const V2 *>(rad + rb);
                         const int zs2 = dm.x * dm.y;
-                        int zi = 0;
+                        T s0 = T(0), st = T(0), s1 = T(0), s2 = T(0);
+                        // whole trips, the last one masked: every load of a trip is
+                        // in flight together
 #pragma unroll 1
-                        for (; zi + U <= n[2]; zi += U) {
+                        for (int zi = 0; zi < n[2]; zi += U) {
                             T v4[U];
                             V2 t4[U];
 #pragma unroll
                             for (int u = 0; u < U; u++) {
-                                v4[u] = vb[(zi + u) * ncol];
-                                t4[u] = r2[(zi + u) * zs2];
+                                const bool ok = zi + u < n[2];
+                                v4[u] = ok ? vb[(zi + u) * ncol] : T(0);
+                                t4[u] = ok ? r2[(zi + u) * zs2] : V2{T(0), T(0)};
                             }
+                            const T zb = (T)(P.hgrid[2] * (a0[2] + zi) - P.range[2] - cen.z);
 #pragma unroll
                             for (int u = 0; u < U; u++) {
-                                const T z = (T)(P.hgrid[2] * (a0[2] + zi + u) - P.range[2] - cen.z);
-                                T c[NLM];
-                                sd_channels<T, NLM>(pz, t4[u].x, t4[u].y, z, c);
-                                const T v = sv * v4[u];
-#pragma unroll
-                                for (int ch = 0; ch < NLM; ch++) acc[ch] = fma(v, c[ch], acc[ch]);
+                                const T z = fma(hzt, T(u), zb);
+                                s0 = fma(v4[u], t4[u].x, s0);
+                                const T vt = v4[u] * t4[u].y;
+                                st = st + vt;
+                                const T tz = vt * z;
+                                s1 = s1 + tz;
+                                s2 = fma(tz, z, s2);
                             }
                         }
-#pragma unroll 1
-                        for (; zi < n[2]; zi++) {
-                            const V2 t2 = r2[zi * zs2];
-                            const T z = (T)(P.hgrid[2] * (a0[2] + zi) - P.range[2] - cen.z);
-                            T c[NLM];
-                            sd_channels<T, NLM>(pz, t2.x, t2.y, z, c);
-                            const T v = sv * vb[zi * ncol];
-#pragma unroll
-                            for (int ch = 0; ch < NLM; ch++) acc[ch] = fma(v, c[ch], acc[ch]);
-                        }
+                        const SdColumn<T> pz = sd_column(x, y);
+                        acc[0] = acc[0] + s0;
+                        acc[1] = fma(pz.cxy, st, acc[1]);
+                        acc[2] = fma(pz.cy, s1, acc[2]);
+                        acc[3] = fma(T(0.315391565252520), fma(pz.q, st, T(2) * s2), acc[3]);
+                        acc[4] = fma(pz.cx, s1, acc[4]);
+                        acc[5] = fma(pz.cxxyy, st, acc[5]);
                     } else {
                         const T *r0 = rad + rb;
                         for (int zi = 0; zi < n[2]; zi += U) {
@@ -291,7 +296,8 @@ __global__ void __launch_bounds__(512) lanczos_zmarch(KParams P)
                 }
 #pragma unroll
                 for (int ch = 0; ch < NLM; ch++) {
-                    const double sm = wave_sum((double)acc[ch]);
+                    double sm = wave_sum((double)acc[ch]);
+                    if constexpr (LAYOUT == 1) sm *= (double)sv;   // the sums used the stored (unscaled) v
                     if (lane == 0 && ch < nlm) A[li * NLM + ch] = sm;
                 }
             }
@@ -416,41 +422,32 @@ __global__ void __launch_bounds__(512) lanczos_zmarch(KParams P)
                         T *bu = Bu + c + zlo * ncol;
                         const int nzr = zhi - zlo + 1;
                         if constexpr (LAYOUT == 1) {
+                            // chi . alpha along the column = t0 al0 + t2 (p0 + p1 z + p2 z^2)
                             using V2 = typename std::conditional<sizeof(T) == 8, double2, float2>::type;
                             const SdColumn<T> pz = sd_column(x, y);
+                            const T C20 = T(0.315391565252520);
+                            const T p0 = fma(pz.cxy, al[1], fma(pz.cxxyy, al[5], C20 * (pz.q * al[3])));
+                            const T p1 = fma(pz.cy, al[2], pz.cx * al[4]);
+                            const T p2 = T(2) * C20 * al[3];
                             const V2 *r2 = reinterpret_cast<const V2 *>(rad + rb);
                             const int zs2 = dm.x * dm.y;
-                            int i = 0;
 #pragma unroll 1
-                            for (; i + U <= nzr; i += U) {
+                            for (int i = 0; i < nzr; i += U) {
                                 T s4[U];
                                 V2 t4[U];
 #pragma unroll
                                 for (int u = 0; u < U; u++) {
-                                    s4[u] = bu[(i + u) * ncol];
-                                    t4[u] = r2[(i + u) * zs2];
+                                    const bool ok = i + u < nzr;
+                                    s4[u] = ok ? bu[(i + u) * ncol] : T(0);
+                                    t4[u] = ok ? r2[(i + u) * zs2] : V2{T(0), T(0)};
                                 }
+                                const T zb = (T)(P.hgrid[2] * (cube_lo[2] + zlo + i) - P.range[2] - cen.z);
 #pragma unroll
                                 for (int u = 0; u < U; u++) {
-                                    const T zc = (T)(P.hgrid[2] * (cube_lo[2] + zlo + i + u) - P.range[2] - cen.z);
-                                    T cv[NLM];
-                                    sd_channels<T, NLM>(pz, t4[u].x, t4[u].y, zc, cv);
-                                    T sum = s4[u];
-#pragma unroll
-                                    for (int ch = 0; ch < NLM; ch++) sum = fma(cv[ch], al[ch], sum);
-                                    bu[(i + u) * ncol] = sum;
+                                    const T z = fma(hzt, T(u), zb);
+                                    const T h = fma(z, fma(z, p2, p1), p0);
+                                    if (i + u < nzr) bu[(i + u) * ncol] = fma(t4[u].y, h, fma(t4[u].x, al[0], s4[u]));
                                 }
-                            }
-#pragma unroll 1
-                            for (; i < nzr; i++) {
-                                const V2 t2 = r2[i * zs2];
-                                const T zc = (T)(P.hgrid[2] * (cube_lo[2] + zlo + i) - P.range[2] - cen.z);
-                                T cv[NLM];
-                                sd_channels<T, NLM>(pz, t2.x, t2.y, zc, cv);
-                                T sum = bu[i * ncol];
-#pragma unroll
-                                for (int ch = 0; ch < NLM; ch++) sum = fma(cv[ch], al[ch], sum);
-                                bu[i * ncol] = sum;
                             }
                         } else {
                             const T *r0 = rad + rb;
