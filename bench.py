@@ -118,7 +118,9 @@ def main():
         "config": {"workload": f"{args.config}: {I.nAtoms}-atom hcp Mg cell, grid {I.npts[0]}x{I.npts[1]}x"
                                f"{I.npts[2]} = {n_nodes} nodes, {I.nloc[0]}^3 cubes, {I.nqp} Lanczos steps, "
                                f"{args.mode} nonlocal, V_eff 0.333",
-                   "nodes": n_nodes, "nqp": I.nqp, "cube": I.nloc[0], "form": "plain FP64"},
+                   "nodes": n_nodes, "nqp": I.nqp, "cube": I.nloc[0], "form": "plain FP64",
+                   "workgroup": I.workgroup[0], "workgroup_fp32": I.workgroup[1],
+                   "autotune_ms": [[round(x, 3) for x in row] for row in I.tune_ms]},
         "roofline": roofline,
         "ql_ms": round(float(k64[:, 1].mean()), 3),
         "fermi_level": lf, "band_energy": eb, "minus_ts": mts,

@@ -73,6 +73,10 @@ typedef struct {
     long long chi_entries;   /* total projector entries over all images */
     int kernel;              /* Lanczos kernel in use: 1 column-marching, 0 generic */
     int kmax;                /* most images meeting one (x,y) column of a cube */
+    int workgroup[2];        /* Lanczos workgroup size per precision (SQ_FP64, SQ_FP32), chosen by
+                                the autotuner at creation */
+    float tune_ms[2][2];     /* autotuner probe times per precision for 512 and 1024 threads
+                                (0 where not timed) */
 } sq_info;
 
 /* Message of the last failing call on this thread ("" if none). */

@@ -47,6 +47,7 @@ class Info(C.Structure):
         ("range", C.c_double * 3), ("delta", C.c_double * 3), ("dV", C.c_double), ("beta", C.c_double),
         ("rcut", C.c_double), ("coeffs", (C.c_double * 11) * 3), ("gamma", (C.c_double * 4) * 8),
         ("cutoff", (C.c_double * 3) * 8), ("chi_entries", C.c_longlong), ("kernel", C.c_int), ("kmax", C.c_int),
+        ("workgroup", C.c_int * 2), ("tune_ms", (C.c_float * 2) * 2),
     ]
 
 

@@ -49,6 +49,8 @@ struct sq_ctx {
     // z-march kernel: eligibility, column-list length, LDS layout, grid per precision
     bool zmarch = false;
     int kmax = 0, zgrid[2] = {0, 0};
+    int zwg[2] = {512, 512};      // z-march workgroup size per precision (512 or 1024 threads)
+    float tune_ms[2][2] = {};     // autotuner probe time per precision and shape (0: 512, 1: 1024)
     size_t zlds = 0;
     int z_tlo = 0, z_tdim = 0, z_toff = 0, z_tcen = 0, z_A = 0, z_alpha = 0, z_red = 0, z_plane = 0;
     hipStream_t stream = nullptr;
@@ -397,6 +399,7 @@ KParams make_params(const sq_ctx *c, int mode, int precision, long begin, long c
         P.lds_plane = c->z_plane;
         P.kmax = c->kmax;
         P.xcd_map = 1;
+        P.zwg = c->zwg[precision];
     }
     P.scratch = c->scratch;
     P.out_a = c->a;
@@ -406,21 +409,57 @@ KParams make_params(const sq_ctx *c, int mode, int precision, long begin, long c
 }
 
 // Grids of the z-march kernel (resident workgroups, a multiple of 8 so the
-// XCD split is even) and the Lanczos scratch sized for the largest grid.
+// XCD split is even), the Lanczos scratch sized for the largest grid, and the
+// workgroup shape per precision: both shapes (512 threads with two columns a
+// thread, 1024 with one) are timed on a short probe -- the first grid's worth
+// of nodes, six Lanczos steps -- and the faster one is kept.  SQ_ZMARCH_WG=512
+// or 1024 fixes the shape instead.
+static int zmarch_grid(sq_ctx *c, int prec, int wg)
+{
+    KParams P = make_params(c, SQ_NL_KB, prec, 0, 1, true);
+    P.zwg = wg;
+    const int occ = zmarch_occupancy(P, prec, c->zlds);
+    return occ <= 0 ? 0 : std::max(8, (c->n_cu * occ) & ~7);
+}
+
 int finish_device_setup(sq_ctx *c)
 {
     const System &S = c->sys;
     int maxgrid = c->grid;
+    int grids[2][2] = {{0, 0}, {0, 0}};
     if (c->zmarch) {
-        for (int prec = 0; prec < 2; prec++) {
-            KParams P = make_params(c, SQ_NL_KB, prec, 0, 1, true);
-            int occ = zmarch_occupancy(P, prec, c->zlds);
-            if (occ <= 0) { c->zmarch = false; break; }
-            c->zgrid[prec] = std::max(8, (c->n_cu * occ) & ~7);
-            maxgrid = std::max(maxgrid, c->zgrid[prec]);
-        }
+        for (int prec = 0; prec < 2 && c->zmarch; prec++)
+            for (int sh = 0; sh < 2; sh++) {
+                grids[prec][sh] = zmarch_grid(c, prec, sh ? 1024 : 512);
+                if (sh == 0 && grids[prec][0] == 0) c->zmarch = false;
+                maxgrid = std::max(maxgrid, grids[prec][sh]);
+            }
     }
     HIP_TRY(hipMalloc(&c->scratch, (size_t)maxgrid * 3 * S.nc() * sizeof(double)));
+    if (!c->zmarch) return SQ_OK;
+    const char *fixed = getenv("SQ_ZMARCH_WG");
+    for (int prec = 0; prec < 2; prec++) {
+        int best = 0;
+        if (fixed) best = atoi(fixed) == 1024 && grids[prec][1] > 0 ? 1 : 0;
+        else if (grids[prec][1] > 0) {
+            for (int sh = 0; sh < 2; sh++) {
+                KParams P = make_params(c, SQ_NL_KB, prec, 0, std::min<long>(grids[prec][sh], S.n_nodes()), true);
+                P.zwg = sh ? 1024 : 512;
+                P.nqp = std::min(P.nqp, 5);
+                const int grid = (int)std::min<long>(grids[prec][sh], std::max<long>(8, (P.node_count + 7) & ~7L));
+                for (int rep = 0; rep < 2; rep++) {   // the first launch loads the code object
+                    HIP_TRY(hipEventRecord(c->ev[0], c->stream));
+                    HIP_TRY(launch_zmarch(P, prec, grid, c->zlds, c->stream));
+                    HIP_TRY(hipEventRecord(c->ev[1], c->stream));
+                }
+                HIP_TRY(hipEventSynchronize(c->ev[1]));
+                HIP_TRY(hipEventElapsedTime(&c->tune_ms[prec][sh], c->ev[0], c->ev[1]));
+            }
+            best = c->tune_ms[prec][1] < c->tune_ms[prec][0] ? 1 : 0;
+        }
+        c->zwg[prec] = best ? 1024 : 512;
+        c->zgrid[prec] = grids[prec][best];
+    }
     return SQ_OK;
 }
 
@@ -529,6 +568,11 @@ int sq_get_info(sq_ctx *c, sq_info *I)
     I->chi_entries = (long long)S.chi.size();
     I->kernel = c->zmarch ? 1 : 0;
     I->kmax = c->kmax;
+    for (int prec = 0; prec < 2; prec++) {
+        I->workgroup[prec] = c->zmarch ? c->zwg[prec] : 1024;
+        I->tune_ms[prec][0] = c->tune_ms[prec][0];
+        I->tune_ms[prec][1] = c->tune_ms[prec][1];
+    }
     return SQ_OK;
 }
 

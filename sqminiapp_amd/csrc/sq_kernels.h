@@ -32,6 +32,7 @@ struct KParams {
     signed char ch_l[kMaxTypesDev][kMaxChannels], ch_m[kMaxTypesDev][kMaxChannels], ch_r[kMaxTypesDev][kMaxChannels];
     int xcd_map;                                  // 1: nodes of one pass split contiguously per XCD
     int layout;                                   // 1: every type has nonlocal l = {0, 2} (s+d channels)
+    int zwg;                                      // z-march workgroup: 512 (1-2 columns per thread) or 1024 (1)
     // stencil weights cx[0..order] per axis (element type = precision)
     const void *cx_ptr, *cy_ptr, *cz_ptr;
     // cube diagonal fl(cdiag + V_eff) on the extended box (element type = precision)
@@ -52,7 +53,6 @@ struct KParams {
 
 hipError_t launch_lanczos(const KParams &P, int precision, int grid, size_t lds, hipStream_t st);
 hipError_t launch_zmarch(const KParams &P, int precision, int grid, size_t lds, hipStream_t st);
-int zmarch_workgroup(int ncol);
 int zmarch_occupancy(const KParams &P, int precision, size_t lds);
 hipError_t launch_apply(const KParams &P, long id, const double *x, double *y, int nonlocal_only, size_t lds,
                         hipStream_t st);

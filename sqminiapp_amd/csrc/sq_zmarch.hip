@@ -55,13 +55,17 @@ __device__ __forceinline__ void sd_channels(const SdColumn<T> &c, T t0, T t2, T 
     for (int ch = 6; ch < NLM; ch++) out[ch] = T(0);
 }
 
-template <typename T, int NLM, int CPT, int MW, int LAYOUT>
-__global__ void __launch_bounds__(512) lanczos_zmarch(KParams P)
+template <typename T, int NLM, int CPT, int MW, int LAYOUT, int WG>
+__global__ void __launch_bounds__(WG) lanczos_zmarch(KParams P)
 {
-    constexpr int WG = 512;
     constexpr int NW = WG / 64;
     constexpr int O = 6;
-    constexpr int U = 8;  // z-planes per trip of the gather and scatter loops
+    // z-planes per trip of the scatter (U) and gather (UG) loops: as many loads
+    // in flight as the register budget of two waves per SIMD allows
+    constexpr bool WL = WG == 1024;   // register-lean shape: 4 waves per SIMD
+    constexpr int U = WL ? (sizeof(T) == 8 ? 4 : 8) : (sizeof(T) == 8 ? 8 : 16);
+    constexpr int UG = U;
+    constexpr int U0 = 4;   // the generic channel layout holds up to four radial factors per point
     extern __shared__ __align__(16) unsigned char smem[];
     int *img_list = reinterpret_cast<int *>(smem);
     int4 *tlo = reinterpret_cast<int4 *>(smem + P.lds_tlo);
@@ -241,18 +245,18 @@ __global__ void __launch_bounds__(512) lanczos_zmarch(KParams P)
                         // whole trips, the last one masked: every load of a trip is
                         // in flight together
 #pragma unroll 1
-                        for (int zi = 0; zi < n[2]; zi += U) {
-                            T v4[U];
-                            V2 t4[U];
+                        for (int zi = 0; zi < n[2]; zi += UG) {
+                            T v4[UG];
+                            V2 t4[UG];
 #pragma unroll
-                            for (int u = 0; u < U; u++) {
+                            for (int u = 0; u < UG; u++) {
                                 const bool ok = zi + u < n[2];
                                 v4[u] = ok ? vb[(zi + u) * ncol] : T(0);
                                 t4[u] = ok ? r2[(zi + u) * zs2] : V2{T(0), T(0)};
                             }
                             const T zb = (T)(P.hgrid[2] * (a0[2] + zi) - P.range[2] - cen.z);
 #pragma unroll
-                            for (int u = 0; u < U; u++) {
+                            for (int u = 0; u < UG; u++) {
                                 const T z = fma(hzt, T(u), zb);
                                 s0 = fma(v4[u], t4[u].x, s0);
                                 const T vt = v4[u] * t4[u].y;
@@ -271,17 +275,17 @@ __global__ void __launch_bounds__(512) lanczos_zmarch(KParams P)
                         acc[5] = fma(pz.cxxyy, st, acc[5]);
                     } else {
                         const T *r0 = rad + rb;
-                        for (int zi = 0; zi < n[2]; zi += U) {
-                            T v4[U], t4[U][4];
+                        for (int zi = 0; zi < n[2]; zi += U0) {
+                            T v4[U0], t4[U0][4];
 #pragma unroll
-                            for (int u = 0; u < U; u++) {
+                            for (int u = 0; u < U0; u++) {
                                 const bool ok = zi + u < n[2];
                                 v4[u] = ok ? vb[(zi + u) * ncol] : T(0);
 #pragma unroll
                                 for (int k = 0; k < 4; k++) t4[u][k] = (ok && k < nl) ? r0[(int64_t)(zi + u) * zs + k] : T(0);
                             }
 #pragma unroll
-                            for (int u = 0; u < U; u++) {
+                            for (int u = 0; u < U0; u++) {
                                 if (zi + u < n[2]) {
                                     const T z = (T)(P.hgrid[2] * (a0[2] + zi + u) - P.range[2] - cen.z);
                                     T c[NLM];
@@ -317,12 +321,18 @@ __global__ void __launch_bounds__(512) lanczos_zmarch(KParams P)
             constexpr int W = 2 * O + 1;
             // stencil weights in vector registers for the march only (read after
             // the barriers above, so they are not held through the projector loops)
+            // (the 1024-thread shape has 128 registers a lane: it reads them from LDS)
             T wreg[3][O + 1];
+            if constexpr (!WL) {
 #pragma unroll
-            for (int d = 0; d < 3; d++)
+                for (int d = 0; d < 3; d++)
 #pragma unroll
-                for (int l = 1; l <= O; l++) wreg[d][l] = s_w[d][l];
-            auto wgt = [&](int d, int l) -> T { return wreg[d][l]; };
+                    for (int l = 1; l <= O; l++) wreg[d][l] = s_w[d][l];
+            }
+            auto wgt = [&](int d, int l) -> T {
+                if constexpr (WL) return s_w[d][l];
+                else return wreg[d][l];
+            };
             double pa = 0.0;
             CSum ca{0.f, 0.f};
             T win[CPT][W];
@@ -503,40 +513,46 @@ __global__ void __launch_bounds__(512) lanczos_zmarch(KParams P)
     }
 }
 
-template <typename T, int NLM, int CPT, int LAYOUT>
+template <typename T, int NLM, int CPT, int LAYOUT, int WG>
 static hipError_t launch_z(const KParams &P, int grid, size_t lds, hipStream_t st)
 {
-    if (P.img_cap <= 128) hipLaunchKernelGGL((lanczos_zmarch<T, NLM, CPT, 4, LAYOUT>), dim3(grid), dim3(512), lds, st, P);
-    else if (P.img_cap <= 256) hipLaunchKernelGGL((lanczos_zmarch<T, NLM, CPT, 8, LAYOUT>), dim3(grid), dim3(512), lds, st, P);
+    if (P.img_cap <= 128) hipLaunchKernelGGL((lanczos_zmarch<T, NLM, CPT, 4, LAYOUT, WG>), dim3(grid), dim3(WG), lds, st, P);
+    else if (P.img_cap <= 256) hipLaunchKernelGGL((lanczos_zmarch<T, NLM, CPT, 8, LAYOUT, WG>), dim3(grid), dim3(WG), lds, st, P);
     else return hipErrorInvalidValue;
     return hipGetLastError();
 }
 
-template <typename T, int NLM, int CPT, int LAYOUT>
+template <typename T, int NLM, int CPT, int LAYOUT, int WG>
 static int occ_z(const KParams &P, size_t lds)
 {
     int n = 0;
     hipError_t e = P.img_cap <= 128
-        ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, lanczos_zmarch<T, NLM, CPT, 4, LAYOUT>, 512, lds)
-        : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, lanczos_zmarch<T, NLM, CPT, 8, LAYOUT>, 512, lds);
+        ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, lanczos_zmarch<T, NLM, CPT, 4, LAYOUT, WG>, WG, lds)
+        : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, lanczos_zmarch<T, NLM, CPT, 8, LAYOUT, WG>, WG, lds);
     return e == hipSuccess ? n : 0;
 }
 
-// Calls f.template run<T, NLM, CPT, LAYOUT>() for the precision, the channel
-// table, the columns per thread and the channel layout of P.
+// Calls f.template run<T, NLM, CPT, LAYOUT, WG>() for the precision, the
+// channel table, the workgroup shape (P.zwg threads; columns per thread from
+// it) and the channel layout of P.
 template <typename R, typename F>
 static R dispatch(const KParams &P, int precision, F &&f)
 {
-    const bool two = P.nloc[0] * P.nloc[1] > 512;
+    const bool wide = P.zwg == 1024;                  // one column per thread
+    const bool two = !wide && P.nloc[0] * P.nloc[1] > 512;
     const bool sd = P.layout == 1 && P.nlm_max <= 6;
+#define SQ_Z_SHAPES(T, NLM, L)                                              \
+    (wide ? f.template run<T, NLM, 1, L, 1024>()                              \
+          : two ? f.template run<T, NLM, 2, L, 512>() : f.template run<T, NLM, 1, L, 512>())
     if (precision == 0) {
-        if (sd) return two ? f.template run<double, 6, 2, 1>() : f.template run<double, 6, 1, 1>();
-        if (P.nlm_max <= 6) return two ? f.template run<double, 6, 2, 0>() : f.template run<double, 6, 1, 0>();
-        return two ? f.template run<double, kMaxChannels, 2, 0>() : f.template run<double, kMaxChannels, 1, 0>();
+        if (sd) return SQ_Z_SHAPES(double, 6, 1);
+        if (P.nlm_max <= 6) return SQ_Z_SHAPES(double, 6, 0);
+        return SQ_Z_SHAPES(double, kMaxChannels, 0);
     }
-    if (sd) return two ? f.template run<float, 6, 2, 1>() : f.template run<float, 6, 1, 1>();
-    if (P.nlm_max <= 6) return two ? f.template run<float, 6, 2, 0>() : f.template run<float, 6, 1, 0>();
-    return two ? f.template run<float, kMaxChannels, 2, 0>() : f.template run<float, kMaxChannels, 1, 0>();
+    if (sd) return SQ_Z_SHAPES(float, 6, 1);
+    if (P.nlm_max <= 6) return SQ_Z_SHAPES(float, 6, 0);
+    return SQ_Z_SHAPES(float, kMaxChannels, 0);
+#undef SQ_Z_SHAPES
 }
 
 struct LaunchOp {
@@ -544,15 +560,15 @@ struct LaunchOp {
     int grid;
     size_t lds;
     hipStream_t st;
-    template <typename T, int NLM, int CPT, int LAYOUT>
-    hipError_t run() { return launch_z<T, NLM, CPT, LAYOUT>(P, grid, lds, st); }
+    template <typename T, int NLM, int CPT, int LAYOUT, int WG>
+    hipError_t run() { return launch_z<T, NLM, CPT, LAYOUT, WG>(P, grid, lds, st); }
 };
 
 struct OccOp {
     const KParams &P;
     size_t lds;
-    template <typename T, int NLM, int CPT, int LAYOUT>
-    int run() { return occ_z<T, NLM, CPT, LAYOUT>(P, lds); }
+    template <typename T, int NLM, int CPT, int LAYOUT, int WG>
+    int run() { return occ_z<T, NLM, CPT, LAYOUT, WG>(P, lds); }
 };
 
 hipError_t launch_zmarch(const KParams &P, int precision, int grid, size_t lds, hipStream_t st)
@@ -561,7 +577,6 @@ hipError_t launch_zmarch(const KParams &P, int precision, int grid, size_t lds, 
     return dispatch<hipError_t>(P, precision, LaunchOp{P, grid, lds, st});
 }
 
-int zmarch_workgroup(int) { return 512; }
 
 int zmarch_occupancy(const KParams &P, int precision, size_t lds)
 {
