@@ -51,6 +51,25 @@ def cpu_baseline(deck_dir, name, n_sample, threads):
 KERNEL_NAMES = {0: "lanczos_kernel (generic)", 1: "lanczos_zmarch (column-marching)"}
 
 
+def scf_wall_time(d, name, max_iter, mode):
+    """SCF wall time on the periodic cell of the same deck: plain FP64 to a
+    density residual of 1e-6, and the FP32 form to 1e-5, its energy judged
+    against FP64 (the metric's 1e-5 Ha/atom)."""
+    import sqminiapp_amd as sq
+    with sq.Context(d, name, device=0, cell=True) as c:
+        I = c.info
+        r64 = c.scf(mode, sq.SQ_FP64, tol=1e-6, max_iter=max_iter)
+        r32 = c.scf(mode, sq.SQ_FP32, tol=1e-5, max_iter=max_iter)
+    keep = ("iterations", "converged", "residual", "wall_s", "sq_s", "poisson_s", "e_total", "lambda_f")
+    f64 = {k: r64[k] for k in keep}
+    f32 = {k: r32[k] for k in keep}
+    return {"workload": f"periodic cell {I.rank_box[3]}x{I.rank_box[4]}x{I.rank_box[5]} = {I.n_nodes} nodes, "
+                        f"BETA {I.beta:g}, deck mixing (Periodic Pulay), LDA-PW, start: superposed atoms",
+            "plain_fp64": f64, "fp32_form": f32,
+            "d_energy_per_atom": abs(r32["e_total"] - r64["e_total"]) / I.nAtoms,
+            "wall_speedup_fp32": r64["wall_s"] / r32["wall_s"] if r32["wall_s"] > 0 else None}
+
+
 def load_traffic(tag):
     path = os.path.join(REPO, "profiles", f"{tag}_pmc.json")
     if not os.path.exists(path):
@@ -69,6 +88,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=192)
     ap.add_argument("--fp32-steps", type=int, default=None)
+    ap.add_argument("--no-scf", action="store_true", help="skip the SCF wall-time measurement")
+    ap.add_argument("--scf-iter", type=int, default=60)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: the SQ node loop would shard across GPUs by 3-D node blocks "
@@ -131,6 +152,8 @@ def main():
         out["fp32_form"] = {"value": round(v32, 1), "unit": "grid-nodes/s", "speedup_vs_fp64": round(v32 / value, 3),
                             "lanczos_ms": round(float(k32[:, 0].mean()), 3),
                             "achieved_GBs": round(b32 / (float(k32[:, 0].mean()) * 1e-3) / 1e9, 1)}
+    if not args.no_scf:
+        out["scf"] = scf_wall_time(d, name, args.scf_iter, mode)
     if not args.no_cpu_baseline:
         threads = max(1, min(16, os.cpu_count() or 1))
         out["cpu_baseline"] = cpu_baseline(d, name, args.cpu_sample, threads)

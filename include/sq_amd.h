@@ -162,6 +162,71 @@ int sq_synchronize(sq_ctx *ctx);
  * chi (k,j,i outer, (l,m) inner, as lsdft_nl.cc:81-123) if chi != NULL. */
 int sq_get_image(sq_ctx *ctx, int c, double *center, int *box, int *atom, int *type, int *entries, double *chi);
 
+/* ------------------------------------------------------------------ SCF
+ * The reference declares its SCF but does not build it (SCF_LSSGQ,
+ * SolvePoisson, Vxc_Calc/Exc_Calc, mix, FormVeffNodal: lsdft.h:93,124,138,
+ * 183,197-204); it builds the pieces: the periodic Poisson operator and KSP
+ * (lsdft_initObjs.cc:249-276,309-388), the pseudocharge (628-817), the
+ * Periodic-Pulay vectors (233-245) and the SCF keywords (lsdft_readfiles.cc:
+ * 219-240).  These entry points assemble them on the periodic cell: the
+ * (N0-1) x (N1-1) x (N2-1) distinct grid points (the atom images repeat every
+ * 2R = (N-1) h), i fastest.  A context used for the SCF must be created with
+ * rank_box = (0, 0, 0, N0-1, N1-1, N2-1), so its nodes are the cell's points. */
+
+typedef struct {
+    int nonlocal_mode;       /* SQ_NL_KB (default) or SQ_NL_REF_COLLAPSED */
+    int precision;           /* SQ_FP64 (default) or SQ_FP32 Lanczos */
+    int max_iter;            /* <= 0: MAXIT_SCF of the deck */
+    double tol;              /* <= 0: TOL_SCF (relative density residual) */
+    double mixing;           /* <= 0: MIXING_PARAMETER */
+    int history;             /* <= 0: MIXING_HISTORY */
+    int pulay_frequency;     /* <= 0: PULAY_FREQUENCY */
+    double tol_poisson;      /* <= 0: TOL_POISSON */
+    const double *rho0;      /* initial density on the cell; NULL: superposed atomic densities */
+    int verbose;             /* 1: one line per iteration on stdout */
+} sq_scf_options;
+
+typedef struct {
+    int iterations, converged, poisson_iterations;
+    double residual;         /* last ||rho_out - rho_in|| / ||rho_out|| */
+    double lambda_f, e_band, minus_ts, e_xc, e_vxc, e_el, e_self, e_total;
+    /* e_total = e_band + e_xc - e_vxc + e_el - e_self + minus_ts (Ha), with
+       e_el = 1/2 int phi (b - rho); energies of the last input density */
+    double wall_s, sq_s, poisson_s;   /* SCF wall time; SQ passes + Fermi level; Poisson solves */
+} sq_scf_result;
+
+enum sq_field {
+    SQ_FIELD_RHO = 0,         /* input density of the last SCF iteration (the converged one on success) */
+    SQ_FIELD_RHO_OUT = 1,     /* mixed density after the last iteration (or rho_out when converged) */
+    SQ_FIELD_PHI = 2,         /* electrostatic potential */
+    SQ_FIELD_VXC = 3,         /* exchange-correlation potential */
+    SQ_FIELD_VEFF = 4,        /* phi + V_xc */
+    SQ_FIELD_B = 5,           /* pseudocharge density (setup) */
+    SQ_FIELD_RHO_ATOMIC = 6   /* superposed isolated-atom density, the default initial guess (setup) */
+};
+
+void sq_default_scf_options(sq_scf_options *opt);
+
+/* SCF_LSSGQ: the SCF loop on the device (V_eff pipeline, SQ pass, density,
+ * Periodic Pulay mixing) until the density residual is below tol. */
+int sq_scf(sq_ctx *ctx, const sq_scf_options *opt, sq_scf_result *result);
+
+/* A field of the cell (sq_field), out[(N0-1)*(N1-1)*(N2-1)]. */
+int sq_get_cell_field(sq_ctx *ctx, int which, double *out);
+
+/* V_eff on the cell (period N-1 in every direction) for the following sq_run:
+ * FormVeffNodal / VeffGlobalToLocal (declared, lsdft.h:197-204). */
+int sq_set_veff_cell(sq_ctx *ctx, const double *veff);
+
+/* Vxc_Calc / Exc_Calc (declared, lsdft.h:138-150) on the device: LDA
+ * (Slater exchange + Perdew-Wang 1992 correlation), eps_xc and v_xc per point. */
+int sq_xc(sq_ctx *ctx, const double *rho, long n, double *eps, double *vxc);
+
+/* SolvePoisson (declared, lsdft.h:93): -(1/4pi) Lap_h phi = f on the cell by
+ * conjugate gradients on the periodic order-2*order stencil to tol (<= 0:
+ * TOL_POISSON), mean(phi) = 0; f must be neutral up to rounding. */
+int sq_poisson(sq_ctx *ctx, const double *f, double tol, double *phi, int *iterations);
+
 #ifdef __cplusplus
 }
 #endif

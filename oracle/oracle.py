@@ -63,6 +63,11 @@ def lib():
         L.sqo_info_size.restype = C.c_int
         L.sqo_apply_h.argtypes = [vp, C.c_int, dp, C.c_double, C.c_int, C.c_int, C.c_int, dp, dp]
         L.sqo_apply_nonlocal.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, dp, dp]
+        L.sqo_set_cell_period.argtypes = [vp, C.c_int]
+        L.sqo_get_valence_density.argtypes = [vp, C.c_int, dp]
+        L.sqo_scf_parameters.argtypes = [vp, dp]
+        L.sqo_pseudocharge.argtypes = [vp, dp, dp, dp]
+        L.sqo_atomic_density.argtypes = [vp, dp, dp]
         L.sqo_lanczos.argtypes = [vp, C.c_int, dp, C.c_double, C.c_int, C.c_int, C.c_int, dp, dp]
         L.sqo_tridiag_nodes_weights.argtypes = [dp, dp, C.c_int, dp, dp, dp, dp]
         L.sqo_nodes_and_weights.argtypes = [vp, C.c_int, dp, C.c_double, ip, C.c_int, dp, dp, dp, dp, C.c_int]
@@ -251,6 +256,49 @@ class System:
         if veff is None or np.isscalar(veff):
             return None, (0.333 if veff is None else float(veff))
         return np.ascontiguousarray(veff, np.float64), 0.0
+
+    # ---- SCF fields on the periodic cell (the (N-1)^3 distinct grid points)
+    def cell(self):
+        I = self.info()
+        return (0, 0, 0, I.npts[0] - 1, I.npts[1] - 1, I.npts[2] - 1)
+
+    def set_cell_period(self, enable=True):
+        """V_eff fields wrap with period N-1 (fields on the periodic cell) instead of N."""
+        lib().sqo_set_cell_period(self._h, 1 if enable else 0)
+
+    def atom_positions(self):
+        I = self.info()
+        pos = np.zeros(3 * I.nAtoms)
+        lib().sqo_get_atoms(self._h, _dp(pos))
+        return pos.reshape(-1, 3)
+
+    def valence_density(self, at):
+        """The isolated-atom valence density on the radial grid (.pot "Valence charge" / 4 pi r^2)."""
+        out = np.zeros(self.info().psd_size[at])
+        lib().sqo_get_valence_density(self._h, at, _dp(out))
+        return out
+
+    def scf_parameters(self):
+        out = np.zeros(6)
+        lib().sqo_scf_parameters(self._h, _dp(out))
+        return dict(beta=out[0], history=int(out[1]), frequency=int(out[2]), tol_scf=out[3], tol_poisson=out[4],
+                    maxit=int(out[5]))
+
+    def pseudocharge(self):
+        """(b on the cell, E_self, neutrality scale)."""
+        I = self.info()
+        b = np.zeros((I.npts[0] - 1) * (I.npts[1] - 1) * (I.npts[2] - 1))
+        es, sc = C.c_double(), C.c_double()
+        _check(lib().sqo_pseudocharge(self._h, _dp(b), C.byref(es), C.byref(sc)), "pseudocharge")
+        return b, es.value, sc.value
+
+    def atomic_density(self):
+        """(superposed isolated-atom density on the cell, scaled to N_e; the scale)."""
+        I = self.info()
+        rho = np.zeros((I.npts[0] - 1) * (I.npts[1] - 1) * (I.npts[2] - 1))
+        sc = C.c_double()
+        _check(lib().sqo_atomic_density(self._h, _dp(rho), C.byref(sc)), "atomic_density")
+        return rho, sc.value
 
     def apply_nonlocal(self, node, x, y, mode=MODE_REF_COLLAPSED):
         """y + V_nl x (Mult_NonlocalVector / Mult_HamiltonianVector accumulate into V2)."""

@@ -47,6 +47,9 @@ typedef struct sqo_sys {
     /* .inpt (lsdft_readfiles.cc:62-102 defaults, 148-331 keywords) */
     int order, bc, nqp, ntype;
     double range[3], beta, rcut, tol_psc;
+    /* SCF keywords (lsdft_readfiles.cc:62-102 defaults, 219-240) */
+    double mix_beta, tol_scf, tol_poisson;
+    int mix_history, pulay_frequency, maxit_scf;
     int npts[3];
     int localPsd[SQO_MAXTYPE];
     char psdName[SQO_MAXTYPE][64];
@@ -68,6 +71,9 @@ typedef struct sqo_sys {
     /* nonlocal overlap for the current rank box */
     int box[6], have_nl;
     sqo_overlap ovl[SQO_MAXTYPE];
+    /* V_eff fields wrap with period N (the DMDA grid, 0) or N-1 (the
+     * distinct points of the periodic cell, 1) */
+    int cell_period;
     /* cube Laplacian in CSR (lsdft_initObjs.cc:393-475) */
     int nc, *rowptr, *colidx;
     double *val;
@@ -288,6 +294,8 @@ static int read_inpt(sqo_sys *S, const char *path)
     if (!f) return SQO_ERR_IO;
     S->ntype = -1; S->order = 6; S->beta = 1000; S->bc = 1; S->nqp = 50; S->rcut = 12.0;
     S->tol_psc = 1e-6;
+    S->mix_beta = 0.3; S->mix_history = 7; S->pulay_frequency = 1; S->tol_scf = 1e-6; S->tol_poisson = 1e-8;
+    S->maxit_scf = 100;
     char str[256];
     do {
         if (fscanf(f, "%255s", str) != 1) break;
@@ -299,6 +307,12 @@ static int read_inpt(sqo_sys *S, const char *path)
         else if (!strcmp(str, "RADIUS_TRUNC:")) { if (fscanf(f, "%lf", &S->rcut) != 1) goto bad; }
         else if (!strcmp(str, "LANCZOS_DEGREE:")) { if (fscanf(f, "%d", &S->nqp) != 1) goto bad; }
         else if (!strcmp(str, "TOL_PSEUDOCHARGE:")) { if (fscanf(f, "%lf", &S->tol_psc) != 1) goto bad; }
+        else if (!strcmp(str, "MIXING_PARAMETER:")) { if (fscanf(f, "%lf", &S->mix_beta) != 1) goto bad; }
+        else if (!strcmp(str, "MIXING_HISTORY:")) { if (fscanf(f, "%d", &S->mix_history) != 1) goto bad; }
+        else if (!strcmp(str, "PULAY_FREQUENCY:")) { if (fscanf(f, "%d", &S->pulay_frequency) != 1) goto bad; }
+        else if (!strcmp(str, "TOL_SCF:")) { if (fscanf(f, "%lf", &S->tol_scf) != 1) goto bad; }
+        else if (!strcmp(str, "TOL_POISSON:")) { if (fscanf(f, "%lf", &S->tol_poisson) != 1) goto bad; }
+        else if (!strcmp(str, "MAXIT_SCF:")) { if (fscanf(f, "%d", &S->maxit_scf) != 1) goto bad; }
         else if (!strcmp(str, "NTYPE:")) {
             if (fscanf(f, "%d", &S->ntype) != 1 || S->ntype < 1 || S->ntype > SQO_MAXTYPE) goto bad;
         } else if (!strcmp(str, "PSEUDOPOTENTIAL_LOCAL:")) {
@@ -896,13 +910,15 @@ static void ws_free(node_ws *W)
 }
 
 /* V_eff on the node's cube: the global field (natural order, i fastest) wrapped
- * periodically in grid index; NULL means the constant veff_const everywhere,
+ * periodically in grid index, with period N or, for fields on the distinct
+ * points of the periodic cell, N-1 (sqo_set_cell_period); NULL means the
+ * constant veff_const everywhere,
  * which is what the reference uses (VecSet(VeffNodal, 0.333), lsdft_scf.cc:71). */
 static void cube_veff(const sqo_sys *S, const double *field, double veff_const, int Kp, int Jp, int Ip, double *out)
 {
     int nx = S->nloc[0], ny = S->nloc[1], nz = S->nloc[2];
     if (!field) { for (int p = 0; p < S->nc; p++) out[p] = veff_const; return; }
-    int Nx = S->npts[0], Ny = S->npts[1], Nz = S->npts[2];
+    int Nx = S->npts[0] - S->cell_period, Ny = S->npts[1] - S->cell_period, Nz = S->npts[2] - S->cell_period;
     int x0 = Ip - (nx - 1) / 2, y0 = Jp - (ny - 1) / 2, z0 = Kp - (nz - 1) / 2;
     for (int k = 0; k < nz; k++) {
         int K = ((z0 + k) % Nz + Nz) % Nz;
@@ -1188,6 +1204,191 @@ typedef struct {
     int lmax[SQO_MAXTYPE], lloc[SQO_MAXTYPE], psd_size[SQO_MAXTYPE], nimages[SQO_MAXTYPE];
 } sqo_info_t;
 
+/* ---------------------------------------------------------- SCF fields
+ *
+ * The reference builds the pieces of its SCF but never assembles them: the
+ * periodic Poisson operator (Create_Laplacian, lsdft_initObjs.cc:309-388),
+ * the pseudocharge construction b = -(1/4pi) Lap_h V_loc
+ * (Calculate_PseudochargeCutoff, 628-817), the local-overlap boxes of every
+ * atom image (GetInfluencingAtoms_Local, 1409-1665) and the isolated-atom
+ * valence density of the .pot file (lsdft_readfiles.cc:1026-1040, spline at
+ * lsdft_initObjs.cc:590).  The functions below assemble them on the
+ * periodic cell: the (N0-1) x (N1-1) x (N2-1) distinct grid points (the
+ * atom images repeat every 2R = (N-1) h, so grid plane N-1 repeats plane 0).
+ */
+void sqo_set_cell_period(sqo_sys *S, int enable) { S->cell_period = enable ? 1 : 0; }
+
+/* beta, history, frequency, TOL_SCF, TOL_POISSON, MAXIT_SCF */
+void sqo_scf_parameters(sqo_sys *S, double *out)
+{
+    out[0] = S->mix_beta; out[1] = S->mix_history; out[2] = S->pulay_frequency;
+    out[3] = S->tol_scf; out[4] = S->tol_poisson; out[5] = S->maxit_scf;
+}
+
+/* V_loc of type at at radius r: the spline inside the radial table, -Z/r past it. */
+static int vloc_at(const sqo_sys *S, int at, double r, double *out)
+{
+    const sqo_psd *P = &S->psd[at];
+    if (r > P->r[P->size - 1]) { *out = -(double)S->noe[at] / r; return SQO_OK; }
+    return sqo_spline_eval(P->r, P->Vloc, P->size, r, P->YD_Vloc, out);
+}
+
+/* Pseudocharge box of one atom image, GetInfluencingAtoms_Local
+ * (lsdft_initObjs.cc:1409-1665): centre index roundf((x0 + R)/h), half-width
+ * ceil(cutoff/h + 0.5). */
+static void psc_box(const sqo_sys *S, int at, const double X[3], int lo[3], int hi[3])
+{
+    for (int d = 0; d < 3; d++) {
+        int off = (int)ceil(S->cutoff[at][d] / S->delta[d] + 0.5);
+        int c = grid_round((X[d] + S->range[d]) / S->delta[d]);
+        lo[d] = c - off;
+        hi[d] = c + off;
+    }
+}
+
+/* b_J = -(1/4pi) Lap_h V_J on the box lo..hi of the image at X, with the
+ * expression and order of lsdft_initObjs.cc:725-735; out is the box, i fastest. */
+static int psc_image(const sqo_sys *S, int at, const double X[3], const int lo[3], const int hi[3], double *out,
+                     double *vout)
+{
+    int o = S->order, n[3], e[3];
+    for (int d = 0; d < 3; d++) { n[d] = hi[d] - lo[d] + 1; e[d] = n[d] + 2 * o; }
+    double *V = dalloc((size_t)e[0] * e[1] * e[2]);
+#define VV(k, j, i) V[((size_t)(k) * e[1] + (j)) * e[0] + (i)]
+    for (int k = 0; k < e[2]; k++)
+        for (int j = 0; j < e[1]; j++)
+            for (int i = 0; i < e[0]; i++) {
+                double x = S->delta[0] * (lo[0] - o + i) - S->range[0] - X[0];
+                double y = S->delta[1] * (lo[1] - o + j) - S->range[1] - X[1];
+                double z = S->delta[2] * (lo[2] - o + k) - S->range[2] - X[2];
+                double r = sqrt(x * x + y * y + z * z);
+                if (vloc_at(S, at, r, &VV(k, j, i))) { free(V); return SQO_ERR_RANGE; }
+            }
+    const double *scx = S->stencil_coeffs[0], *scy = S->stencil_coeffs[1], *scz = S->stencil_coeffs[2];
+    for (int k = o; k < n[2] + o; k++)
+        for (int j = o; j < n[1] + o; j++)
+            for (int i = o; i < n[0] + o; i++) {
+                double v = VV(k, j, i) * (scx[0] + scy[0] + scz[0]);
+                for (int a = 1; a <= o; a++)
+                    v += (VV(k, j, i - a) + VV(k, j, i + a)) * scx[a] +
+                         (VV(k, j - a, i) + VV(k, j + a, i)) * scy[a] +
+                         (VV(k - a, j, i) + VV(k + a, j, i)) * scz[a];
+                size_t q = ((size_t)(k - o) * n[1] + (j - o)) * n[0] + (i - o);
+                out[q] = v;
+                if (vout) vout[q] = VV(k, j, i);
+            }
+#undef VV
+    free(V);
+    return SQO_OK;
+}
+
+/* The pseudocharge on the periodic cell, b[(k*(N1-1)+j)*(N0-1)+i]: the sum
+ * over atoms (no phantom) and their images (shifts 2R, image range
+ * ceil(cutoff/R) as lsdft_initObjs.cc:1490-1499) of b_J on the part of each
+ * box inside the cell, then scaled by -N_e/(sum b dV) so the cell is neutral
+ * (the truncation at the cutoff leaves |sum b dV + N_e|/N_e <= TOL_PSEUDOCHARGE).
+ * eself = (scale/2) sum_J sum_box b_J V_J dV over each atom's whole box. */
+int sqo_pseudocharge(sqo_sys *S, double *b, double *eself, double *scale)
+{
+    const int G[3] = {S->npts[0] - 1, S->npts[1] - 1, S->npts[2] - 1};
+    const size_t ng = (size_t)G[0] * G[1] * G[2];
+    memset(b, 0, sizeof(double) * ng);
+    double es = 0.0;
+    for (int at = 0; at < S->ntype; at++) {
+        int Imax[3];
+        image_range(S, at, Imax);
+        for (int p = S->startPos[at] / 3; p <= S->endPos[at] / 3; p++) {
+            const double *A = &S->atompos[3 * p];
+            for (int PP = -Imax[0]; PP <= Imax[0]; PP++)
+                for (int QQ = -Imax[1]; QQ <= Imax[1]; QQ++)
+                    for (int RR = -Imax[2]; RR <= Imax[2]; RR++) {
+                        double X[3] = {A[0] + PP * 2.0 * S->range[0], A[1] + QQ * 2.0 * S->range[1],
+                                       A[2] + RR * 2.0 * S->range[2]};
+                        int lo[3], hi[3], clo[3], chi[3], empty = 0;
+                        psc_box(S, at, X, lo, hi);
+                        for (int d = 0; d < 3; d++) {
+                            clo[d] = lo[d] > 0 ? lo[d] : 0;
+                            chi[d] = hi[d] < G[d] - 1 ? hi[d] : G[d] - 1;
+                            empty |= clo[d] > chi[d];
+                        }
+                        const int self = PP == 0 && QQ == 0 && RR == 0;
+                        if (empty && !self) continue;
+                        int n[3] = {hi[0] - lo[0] + 1, hi[1] - lo[1] + 1, hi[2] - lo[2] + 1};
+                        size_t nb = (size_t)n[0] * n[1] * n[2];
+                        double *bj = dalloc(nb), *vj = dalloc(nb);
+                        if (psc_image(S, at, X, lo, hi, bj, vj)) { free(bj); free(vj); return SQO_ERR_RANGE; }
+                        if (self)
+                            for (size_t q = 0; q < nb; q++) es += bj[q] * vj[q];
+                        if (!empty)
+                            for (int k = clo[2]; k <= chi[2]; k++)
+                                for (int j = clo[1]; j <= chi[1]; j++)
+                                    for (int i = clo[0]; i <= chi[0]; i++)
+                                        b[((size_t)k * G[1] + j) * G[0] + i] +=
+                                            bj[((size_t)(k - lo[2]) * n[1] + (j - lo[1])) * n[0] + (i - lo[0])];
+                        free(bj);
+                        free(vj);
+                    }
+        }
+    }
+    double tot = 0.0;
+    for (size_t q = 0; q < ng; q++) tot += b[q];
+    tot *= S->dV;
+    const double sc = -(double)S->noetot / tot;
+    for (size_t q = 0; q < ng; q++) b[q] *= sc;
+    *scale = sc;
+    *eself = 0.5 * es * S->dV * sc;
+    return SQO_OK;
+}
+
+/* Superposition of the isolated-atom valence densities (the .pot "Valence
+ * charge" block over 4 pi r^2, lsdft_readfiles.cc:1026-1040) on the same
+ * boxes, scaled to N_e electrons: the initial SCF density. */
+int sqo_atomic_density(sqo_sys *S, double *rho, double *scale)
+{
+    const int G[3] = {S->npts[0] - 1, S->npts[1] - 1, S->npts[2] - 1};
+    const size_t ng = (size_t)G[0] * G[1] * G[2];
+    memset(rho, 0, sizeof(double) * ng);
+    for (int at = 0; at < S->ntype; at++) {
+        const sqo_psd *P = &S->psd[at];
+        int Imax[3];
+        image_range(S, at, Imax);
+        for (int p = S->startPos[at] / 3; p <= S->endPos[at] / 3; p++) {
+            const double *A = &S->atompos[3 * p];
+            for (int PP = -Imax[0]; PP <= Imax[0]; PP++)
+                for (int QQ = -Imax[1]; QQ <= Imax[1]; QQ++)
+                    for (int RR = -Imax[2]; RR <= Imax[2]; RR++) {
+                        double X[3] = {A[0] + PP * 2.0 * S->range[0], A[1] + QQ * 2.0 * S->range[1],
+                                       A[2] + RR * 2.0 * S->range[2]};
+                        int lo[3], hi[3];
+                        psc_box(S, at, X, lo, hi);
+                        for (int d = 0; d < 3; d++) {
+                            if (lo[d] < 0) lo[d] = 0;
+                            if (hi[d] > G[d] - 1) hi[d] = G[d] - 1;
+                        }
+                        for (int k = lo[2]; k <= hi[2]; k++)
+                            for (int j = lo[1]; j <= hi[1]; j++)
+                                for (int i = lo[0]; i <= hi[0]; i++) {
+                                    double x = S->delta[0] * i - S->range[0] - X[0];
+                                    double y = S->delta[1] * j - S->range[1] - X[1];
+                                    double z = S->delta[2] * k - S->range[2] - X[2];
+                                    double r = sqrt(x * x + y * y + z * z), v = 0.0;
+                                    if (r <= P->r[P->size - 1] &&
+                                        sqo_spline_eval(P->r, P->uu, P->size, r, P->YD_uu, &v))
+                                        return SQO_ERR_RANGE;
+                                    rho[((size_t)k * G[1] + j) * G[0] + i] += v;
+                                }
+                    }
+        }
+    }
+    double tot = 0.0;
+    for (size_t q = 0; q < ng; q++) tot += rho[q];
+    tot *= S->dV;
+    const double sc = (double)S->noetot / tot;
+    for (size_t q = 0; q < ng; q++) rho[q] *= sc;
+    *scale = sc;
+    return SQO_OK;
+}
+
 void sqo_info(sqo_sys *S, sqo_info_t *I)
 {
     memset(I, 0, sizeof *I);
@@ -1255,6 +1456,13 @@ int sqo_get_chi(sqo_sys *S, int at, int c, double *chi, int *lId, int *mId, int 
 int sqo_get_ip_index(sqo_sys *S, int *out) { memcpy(out, S->IP_index, sizeof(int) * S->nIP); return S->nIP; }
 
 /* Atom positions of the .ion file (3*nAtoms) */
+int sqo_get_valence_density(sqo_sys *S, int at, double *out)
+{
+    if (at < 0 || at >= S->ntype) return -1;
+    memcpy(out, S->psd[at].uu, sizeof(double) * S->psd[at].size);
+    return S->psd[at].size;
+}
+
 int sqo_get_atoms(sqo_sys *S, double *pos) { memcpy(pos, S->atompos, sizeof(double) * 3 * S->nAtoms); return S->nAtoms; }
 
 /* Dense cube Laplacian values (CSR) for brute-force checks. */

@@ -24,7 +24,11 @@ STATUS = {0: "SQ_OK", 1: "SQ_ERR_IO", 2: "SQ_ERR_FORMAT", 3: "SQ_ERR_RANGE", 4: 
 # every symbol include/sq_amd.h declares
 EXPORTS = ["sq_last_error", "sq_default_options", "sq_create_from_deck", "sq_destroy", "sq_get_info", "sq_set_veff",
            "sq_run", "sq_get_nodes_and_weights", "sq_get_lanczos", "sq_nodes_and_weights", "sq_last_timing",
-           "sq_apply_h", "sq_apply_nonlocal", "sq_tridiag_nodes_weights", "sq_density_fermi_energy", "sq_synchronize", "sq_get_image"]
+           "sq_apply_h", "sq_apply_nonlocal", "sq_tridiag_nodes_weights", "sq_density_fermi_energy", "sq_synchronize",
+           "sq_get_image", "sq_default_scf_options", "sq_scf", "sq_get_cell_field", "sq_set_veff_cell", "sq_xc",
+           "sq_poisson"]
+
+FIELDS = {"rho": 0, "rho_out": 1, "phi": 2, "vxc": 3, "veff": 4, "b": 5, "rho_atomic": 6}
 
 
 class SQError(RuntimeError):
@@ -36,6 +40,27 @@ class SQError(RuntimeError):
 class Options(C.Structure):
     _fields_ = [("phantom_atom", C.c_int), ("phantom_pos", C.c_double * 3), ("rank_box", C.c_int * 6),
                 ("nqp_override", C.c_int), ("device", C.c_int)]
+
+
+class ScfOptions(C.Structure):
+    _fields_ = [("nonlocal_mode", C.c_int), ("precision", C.c_int), ("max_iter", C.c_int), ("tol", C.c_double),
+                ("mixing", C.c_double), ("history", C.c_int), ("pulay_frequency", C.c_int),
+                ("tol_poisson", C.c_double), ("rho0", C.POINTER(C.c_double)), ("verbose", C.c_int)]
+
+
+class ScfResult(C.Structure):
+    _fields_ = [("iterations", C.c_int), ("converged", C.c_int), ("poisson_iterations", C.c_int),
+                ("residual", C.c_double), ("lambda_f", C.c_double), ("e_band", C.c_double),
+                ("minus_ts", C.c_double), ("e_xc", C.c_double), ("e_vxc", C.c_double), ("e_el", C.c_double),
+                ("e_self", C.c_double), ("e_total", C.c_double), ("wall_s", C.c_double), ("sq_s", C.c_double),
+                ("poisson_s", C.c_double)]
+
+
+def deck_grid(directory, name):
+    """FD_GRID of a deck (N0, N1, N2)."""
+    toks = open(os.path.join(str(directory), f"{name}.inpt")).read().split()
+    i = toks.index("FD_GRID:")
+    return tuple(int(t) for t in toks[i + 1:i + 4])
 
 
 class Info(C.Structure):
@@ -76,6 +101,12 @@ def lib():
     L.sq_last_timing.argtypes = [vp, dp, dp]
     L.sq_apply_h.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, dp, dp]
     L.sq_apply_nonlocal.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, dp, dp]
+    L.sq_default_scf_options.argtypes = [C.POINTER(ScfOptions)]
+    L.sq_scf.argtypes = [vp, C.POINTER(ScfOptions), C.POINTER(ScfResult)]
+    L.sq_get_cell_field.argtypes = [vp, C.c_int, dp]
+    L.sq_set_veff_cell.argtypes = [vp, dp]
+    L.sq_xc.argtypes = [vp, dp, lg, dp, dp]
+    L.sq_poisson.argtypes = [vp, dp, C.c_double, dp, C.POINTER(C.c_int)]
     L.sq_tridiag_nodes_weights.argtypes = [vp, dp, dp, C.c_int, lg, dp, dp, dp, dp]
     L.sq_density_fermi_energy.argtypes = [vp, C.c_double, C.c_double, dp, dp, dp, dp]
     L.sq_synchronize.argtypes = [vp]
@@ -97,7 +128,12 @@ class Context:
     """One SQ problem on one device (or host-only with device=-1)."""
 
     def __init__(self, directory, name, *, phantom_atom=False, phantom_pos=(0.0, 0.0, 0.0), rank_box=None,
-                 nqp=0, device=0):
+                 nqp=0, device=0, cell=False):
+        """cell=True: the node box is the periodic cell (the (N-1)^3 distinct grid
+        points), as the SCF entry points require."""
+        if cell:
+            N = deck_grid(directory, name)
+            rank_box = (0, 0, 0, N[0] - 1, N[1] - 1, N[2] - 1)
         L = lib()
         opt = Options()
         L.sq_default_options(C.byref(opt))
@@ -205,6 +241,51 @@ class Context:
             raise ValueError("vector size differs from the cube")
         _check(lib().sq_apply_nonlocal(self._h, int(mode), int(K), int(J), int(I_), _dp(x), _dp(y)))
         return y
+
+    # ---- SCF on the periodic cell
+    def cell_shape(self):
+        """(z, y, x) shape of the cell fields."""
+        I = self.info
+        return (I.npts[2] - 1, I.npts[1] - 1, I.npts[0] - 1)
+
+    def scf(self, mode=SQ_NL_KB, precision=SQ_FP64, max_iter=0, tol=0.0, mixing=0.0, history=0,
+            pulay_frequency=0, tol_poisson=0.0, rho0=None, verbose=False):
+        o = ScfOptions()
+        lib().sq_default_scf_options(C.byref(o))
+        o.nonlocal_mode, o.precision, o.max_iter = int(mode), int(precision), int(max_iter)
+        o.tol, o.mixing, o.history, o.pulay_frequency = float(tol), float(mixing), int(history), int(pulay_frequency)
+        o.tol_poisson, o.verbose = float(tol_poisson), int(bool(verbose))
+        keep = None
+        if rho0 is not None:
+            keep = np.ascontiguousarray(rho0, np.float64).ravel()
+            o.rho0 = _dp(keep)
+        r = ScfResult()
+        _check(lib().sq_scf(self._h, C.byref(o), C.byref(r)))
+        return {k: getattr(r, k) for k, _ in ScfResult._fields_}
+
+    def cell_field(self, name):
+        out = np.zeros(int(np.prod(self.cell_shape())))
+        _check(lib().sq_get_cell_field(self._h, FIELDS[name], _dp(out)))
+        return out.reshape(self.cell_shape())
+
+    def set_veff_cell(self, veff):
+        v = np.ascontiguousarray(veff, np.float64).ravel()
+        if v.size != int(np.prod(self.cell_shape())):
+            raise ValueError("V_eff must be a field on the periodic cell")
+        _check(lib().sq_set_veff_cell(self._h, _dp(v)))
+
+    def xc(self, rho):
+        r = np.ascontiguousarray(rho, np.float64).ravel()
+        eps, v = np.zeros_like(r), np.zeros_like(r)
+        _check(lib().sq_xc(self._h, _dp(r), r.size, _dp(eps), _dp(v)))
+        return eps.reshape(np.shape(rho)), v.reshape(np.shape(rho))
+
+    def poisson(self, f, tol=0.0):
+        fr = np.ascontiguousarray(f, np.float64).ravel()
+        phi = np.zeros_like(fr)
+        its = C.c_int()
+        _check(lib().sq_poisson(self._h, _dp(fr), float(tol), _dp(phi), C.byref(its)))
+        return phi.reshape(np.shape(f)), its.value
 
     def tridiag_nodes_weights(self, diag, subdiag):
         diag = np.ascontiguousarray(diag, np.float64)

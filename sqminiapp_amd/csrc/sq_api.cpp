@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -79,6 +80,17 @@ struct sq_ctx {
     long last_begin = 0, last_count = 0;
     bool have_run = false;
     double t_lanczos = 0, t_ql = 0;
+    // SCF state on the periodic cell (allocated by the first SCF call)
+    struct Scf {
+        long ng = 0;
+        int hist = 0;
+        double *rho = nullptr, *rho_out = nullptr, *b = nullptr, *phi = nullptr, *eps = nullptr, *vxc = nullptr,
+               *veff = nullptr, *r = nullptr, *p = nullptr, *Ap = nullptr, *f = nullptr, *x_prev = nullptr,
+               *f_prev = nullptr, *dX = nullptr, *dF = nullptr, *g = nullptr, *s = nullptr, *part = nullptr;
+        bool have_b = false;
+        double eself = 0, b_scale = 0;
+        std::vector<double> host_b;
+    } scf;
 };
 
 namespace {
@@ -97,6 +109,11 @@ void free_ctx(sq_ctx *c)
     void *ptrs[] = {c->img_dimp, c->rad64, c->rad32, c->img_rad, c->img_cen, c->chi64, c->chi32, c->img_lo, c->img_dim, c->img_off, c->img_atom, c->img_type, c->gamma,
                     c->cov_off, c->cov_img, c->diag64, c->diag32, c->c64, c->c32, c->scratch, c->a, c->b,
                     c->lam, c->w, c->lmin, c->lmax, c->qlws, c->status, c->partial, c->partial2, c->out};
+    void *scf_ptrs[] = {c->scf.rho, c->scf.rho_out, c->scf.b, c->scf.phi, c->scf.eps, c->scf.vxc, c->scf.veff, c->scf.r,
+                        c->scf.p, c->scf.Ap, c->scf.f, c->scf.x_prev, c->scf.f_prev, c->scf.dX, c->scf.dF, c->scf.g,
+                        c->scf.s, c->scf.part};
+    for (void *p : scf_ptrs)
+        if (p) (void)hipFree(p);
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (hipEvent_t e : c->ev)
@@ -755,10 +772,11 @@ int sq_tridiag_nodes_weights(sq_ctx *c, const double *diag, const double *subdia
     return rc;
 }
 
-int sq_density_fermi_energy(sq_ctx *c, double nelec, double beta, double *lambda_f, double *rho, double *e_band,
-                            double *minus_ts)
+// Fermi level by bisection on the device electron count, then rho (into the
+// device array drho when given), E_band and -TS.
+static int density_device(sq_ctx *c, double nelec, double beta, double *lambda_f, double *drho, double *e_band,
+                          double *minus_ts)
 {
-    if (!c || !lambda_f || !e_band || !minus_ts) return set_err(SQ_ERR_ARG, "null argument");
     const System &S = c->sys;
     const long nn = S.n_nodes();
     if (!c->have_run || c->last_begin != 0 || c->last_count != nn)
@@ -793,18 +811,395 @@ int sq_density_fermi_energy(sq_ctx *c, double nelec, double beta, double *lambda
     }
     const double lf = 0.5 * (lo + hi);
     double res[2];
-    double *drho = nullptr;
-    if (rho) HIP_TRY(hipMalloc((void **)&drho, nn * sizeof(double)));
-    hipError_t e1 = launch_density_energy(c->lam, c->w, nn, S.nqp, beta, lf, 2.0 / S.dV, drho, c->partial, c->partial2,
-                                          kRedBlocks, c->out, c->stream);
-    hipError_t e2 = e1 == hipSuccess ? hipMemcpyAsync(res, c->out, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream) : e1;
-    hipError_t e3 = e2 == hipSuccess ? hipStreamSynchronize(c->stream) : e2;
-    if (e3 == hipSuccess && rho) e3 = hipMemcpy(rho, drho, nn * sizeof(double), hipMemcpyDeviceToHost);
-    if (drho) (void)hipFree(drho);
-    if (e3 != hipSuccess) return set_err(SQ_ERR_DEVICE, std::string("density step: ") + hipGetErrorString(e3));
+    HIP_TRY(launch_density_energy(c->lam, c->w, nn, S.nqp, beta, lf, 2.0 / S.dV, drho, c->partial, c->partial2,
+                                  kRedBlocks, c->out, c->stream));
+    HIP_TRY(hipMemcpyAsync(res, c->out, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
     *lambda_f = lf;
     *e_band = res[0];
     *minus_ts = res[1];
+    return SQ_OK;
+}
+
+int sq_density_fermi_energy(sq_ctx *c, double nelec, double beta, double *lambda_f, double *rho, double *e_band,
+                            double *minus_ts)
+{
+    if (!c || !lambda_f || !e_band || !minus_ts) return set_err(SQ_ERR_ARG, "null argument");
+    if (!c->stream) return set_err(SQ_ERR_STATE, "host-only context (created with device < 0)");
+    const long nn = c->sys.n_nodes();
+    double *drho = nullptr;
+    if (rho) HIP_TRY(hipMalloc((void **)&drho, nn * sizeof(double)));
+    int rc = density_device(c, nelec, beta, lambda_f, drho, e_band, minus_ts);
+    if (!rc && rho && hipMemcpy(rho, drho, nn * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
+        rc = set_err(SQ_ERR_DEVICE, "density copy failed");
+    if (drho) (void)hipFree(drho);
+    return rc;
+}
+
+// ================================================================= SCF
+// The reference declares its SCF (SCF_LSSGQ, SolvePoisson, Vxc_Calc/Exc_Calc,
+// mix: src/lsdft.h:93,124,138,183,197-204) and builds only the pieces; the
+// loop here runs on the periodic cell -- the (N-1)^3 distinct grid points,
+// the context's rank box -- with every field on the device:
+//   V_xc, eps_xc  <- LDA (Slater + PW92) of rho_in
+//   phi           <- -(1/4pi) Lap_h phi = rho_in + b by conjugate gradients on
+//                    the periodic -(1/2) Lap_h (warm start, mean zero)
+//   V_eff = phi + V_xc -> the node cubes' diagonal -> SQ pass -> rho_out
+//   rho_in        <- Periodic Pulay mix of (rho_in, rho_out - rho_in)
+// until ||rho_out - rho_in|| / ||rho_out|| < TOL_SCF, with the Harris-Foulkes
+// energy E = E_band + E_xc - int V_xc rho + 1/2 int phi (b - rho) - E_self - TS.
+
+static int cell_dims(const sq_ctx *c, int G[3])
+{
+    const System &S = c->sys;
+    for (int d = 0; d < 3; d++) {
+        G[d] = S.npts[d] - 1;
+        if (S.box[d] != 0 || S.box[3 + d] != G[d])
+            return set_err(SQ_ERR_STATE, "the SCF runs on the periodic cell: create the context with rank_box = "
+                                         "(0, 0, 0, N0-1, N1-1, N2-1)");
+    }
+    return SQ_OK;
+}
+
+static StencilCoeffs stencil_of(const System &S)
+{
+    StencilCoeffs sc;
+    memset(&sc, 0, sizeof sc);
+    for (int d = 0; d < 3; d++)
+        for (int a = 0; a <= S.order && a < 11; a++) sc.w[d][a] = S.coeff[d][a];
+    return sc;
+}
+
+static int scf_alloc(sq_ctx *c, int hist)
+{
+    auto &F = c->scf;
+    const long ng = c->sys.n_nodes();
+    if (F.rho && F.ng == ng && F.hist >= hist) return SQ_OK;
+    double **all[] = {&F.rho, &F.rho_out, &F.b, &F.phi, &F.eps, &F.vxc, &F.veff, &F.r, &F.p, &F.Ap, &F.f, &F.x_prev,
+                      &F.f_prev, &F.dX, &F.dF, &F.g, &F.s, &F.part};
+    for (double **q : all) {
+        if (*q) (void)hipFree(*q);
+        *q = nullptr;
+    }
+    double **vecs[] = {&F.rho, &F.rho_out, &F.b, &F.phi, &F.eps, &F.vxc, &F.veff, &F.r, &F.p, &F.Ap, &F.f, &F.x_prev, &F.f_prev};
+    for (double **v : vecs) {
+        HIP_TRY(hipMalloc((void **)v, ng * sizeof(double)));
+        HIP_TRY(hipMemset(*v, 0, ng * sizeof(double)));
+    }
+    HIP_TRY(hipMalloc((void **)&F.dX, std::max(1, hist) * ng * sizeof(double)));
+    HIP_TRY(hipMalloc((void **)&F.dF, std::max(1, hist) * ng * sizeof(double)));
+    HIP_TRY(hipMalloc((void **)&F.g, std::max(1, hist) * sizeof(double)));
+    HIP_TRY(hipMalloc((void **)&F.s, 16 * sizeof(double)));
+    HIP_TRY(hipMalloc((void **)&F.part, kRedBlocks * sizeof(double)));
+    F.ng = ng;
+    F.hist = hist;
+    if (F.have_b) HIP_TRY(hipMemcpy(F.b, F.host_b.data(), ng * sizeof(double), hipMemcpyHostToDevice));
+    return SQ_OK;
+}
+
+static int scf_pseudocharge(sq_ctx *c)
+{
+    auto &F = c->scf;
+    if (!F.have_b) {
+        Status st = build_pseudocharge(c->sys, F.host_b, F.eself, F.b_scale);
+        if (!st.ok()) return set_err(st.code, st.msg);
+        F.have_b = true;
+        if (F.b) HIP_TRY(hipMemcpy(F.b, F.host_b.data(), F.ng * sizeof(double), hipMemcpyHostToDevice));
+    }
+    return SQ_OK;
+}
+
+// <a, b> (or sum a when b is null) into slot of F.s, read back to host when out != nullptr
+static int scf_dot(sq_ctx *c, const double *a, const double *b, int slot, double *out)
+{
+    auto &F = c->scf;
+    HIP_TRY(launch_dot(a, b, F.ng, 1.0, F.part, kRedBlocks, F.s + slot, c->stream));
+    if (out) {
+        HIP_TRY(hipMemcpyAsync(out, F.s + slot, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(hipStreamSynchronize(c->stream));
+    }
+    return SQ_OK;
+}
+
+// Solve -(1/4pi) Lap_h phi = f (f on the device, neutral up to rounding) into
+// F.phi by conjugate gradients on A = -(1/2) Lap_h, A phi = 2 pi f, from the
+// current F.phi; stops at ||r|| <= tol ||2 pi f|| (checked every 4 steps, the
+// scalars of a step stay on the device) and removes the mean of phi.
+static int scf_poisson(sq_ctx *c, const double *f, double tol, int max_it, int *iters)
+{
+    auto &F = c->scf;
+    const System &S = c->sys;
+    int G[3];
+    if (int rc = cell_dims(c, G)) return rc;
+    const StencilCoeffs sc = stencil_of(S);
+    const long n = F.ng;
+    int rc;
+    double mean = 0.0;
+    // right-hand side without its (rounding-level) mean: F.r = 2 pi (f - mean f)
+    if ((rc = scf_dot(c, f, nullptr, 4, &mean))) return rc;
+    mean /= (double)n;
+    HIP_TRY(launch_axpbypgz(2.0 * M_PI, f, 0.0, F.r, -2.0 * M_PI * mean, nullptr, n, c->stream));
+    double bnorm2 = 0.0;
+    if ((rc = scf_dot(c, F.r, F.r, 5, &bnorm2))) return rc;
+    const double stop = tol * tol * bnorm2;
+    // r = 2 pi f - A phi ; p = r ; s0 = <r,r>
+    HIP_TRY(launch_lap_periodic(F.phi, F.Ap, G, S.order, sc, c->stream));
+    HIP_TRY(launch_axpbypgz(-1.0, F.Ap, 1.0, F.r, 0.0, nullptr, n, c->stream));
+    HIP_TRY(hipMemcpyAsync(F.p, F.r, n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    if ((rc = scf_dot(c, F.r, F.r, 0, nullptr))) return rc;
+    int it = 0;
+    double rr = 0.0;
+    HIP_TRY(hipMemcpyAsync(&rr, F.s, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    while (rr > stop && it < max_it) {
+        for (int k = 0; k < 4 && it < max_it; k++, it++) {
+            HIP_TRY(launch_lap_periodic(F.p, F.Ap, G, S.order, sc, c->stream));
+            if ((rc = scf_dot(c, F.p, F.Ap, 1, nullptr))) return rc;
+            HIP_TRY(launch_cg_xr(F.phi, F.r, F.p, F.Ap, F.s, n, c->stream));
+            if ((rc = scf_dot(c, F.r, F.r, 2, nullptr))) return rc;
+            HIP_TRY(launch_cg_p(F.p, F.r, F.s, n, c->stream));
+        }
+        HIP_TRY(hipMemcpyAsync(&rr, F.s, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(hipStreamSynchronize(c->stream));
+    }
+    if (!(rr <= stop)) return set_err(SQ_ERR_RANGE, "Poisson: conjugate gradients did not reach TOL_POISSON");
+    if ((rc = scf_dot(c, F.phi, nullptr, 4, &mean))) return rc;
+    mean /= (double)n;
+    HIP_TRY(launch_axpbypgz(1.0, F.phi, 0.0, F.phi, -mean, nullptr, n, c->stream));
+    if (iters) *iters = it;
+    return SQ_OK;
+}
+
+// Gaussian elimination with partial pivoting on the m x m system (row major).
+static bool small_solve(std::vector<double> A, std::vector<double> &x, int m)
+{
+    for (int col = 0; col < m; col++) {
+        int piv = col;
+        for (int r = col + 1; r < m; r++)
+            if (std::fabs(A[r * m + col]) > std::fabs(A[piv * m + col])) piv = r;
+        if (A[piv * m + col] == 0.0) return false;
+        if (piv != col) {
+            for (int k = 0; k < m; k++) std::swap(A[col * m + k], A[piv * m + k]);
+            std::swap(x[col], x[piv]);
+        }
+        for (int r = col + 1; r < m; r++) {
+            const double f = A[r * m + col] / A[col * m + col];
+            for (int k = col; k < m; k++) A[r * m + k] -= f * A[col * m + k];
+            x[r] -= f * x[col];
+        }
+    }
+    for (int r = m - 1; r >= 0; r--) {
+        double v = x[r];
+        for (int k = r + 1; k < m; k++) v -= A[r * m + k] * x[k];
+        x[r] = v / A[r * m + r];
+    }
+    return true;
+}
+
+void sq_default_scf_options(sq_scf_options *o)
+{
+    if (!o) return;
+    memset(o, 0, sizeof *o);
+    o->nonlocal_mode = SQ_NL_KB;
+    o->precision = SQ_FP64;
+}
+
+int sq_scf(sq_ctx *c, const sq_scf_options *opt_in, sq_scf_result *res)
+{
+    if (!c || !res) return set_err(SQ_ERR_ARG, "null argument");
+    if (!c->stream) return set_err(SQ_ERR_STATE, "host-only context (created with device < 0)");
+    sq_scf_options opt;
+    sq_default_scf_options(&opt);
+    if (opt_in) opt = *opt_in;
+    const System &S = c->sys;
+    int G[3], rc;
+    if ((rc = cell_dims(c, G))) return rc;
+    if (S.xc != "LDA" && S.xc != "LDA_PW") return set_err(SQ_ERR_ARG, "EXCHANGE_CORRELATION " + S.xc + " is not LDA_PW");
+    const int maxit = opt.max_iter > 0 ? opt.max_iter : S.maxit_scf;
+    const double tol = opt.tol > 0 ? opt.tol : S.tol_scf;
+    const double beta = opt.mixing > 0 ? opt.mixing : S.mix_beta;
+    const int m = opt.history > 0 ? opt.history : S.mix_history;
+    const int pf = opt.pulay_frequency > 0 ? opt.pulay_frequency : std::max(1, S.pulay_frequency);
+    const double tolp = opt.tol_poisson > 0 ? opt.tol_poisson : S.tol_poisson;
+    HIP_TRY(hipSetDevice(c->device));
+    const auto t0 = std::chrono::steady_clock::now();
+    if ((rc = scf_alloc(c, m)) || (rc = scf_pseudocharge(c))) return rc;
+    auto &F = c->scf;
+    const long n = F.ng;
+    const double dV = S.dV;
+    if (opt.rho0) HIP_TRY(hipMemcpy(F.rho, opt.rho0, n * sizeof(double), hipMemcpyHostToDevice));
+    else {
+        std::vector<double> rho0;
+        double sc;
+        Status st = build_atomic_density(S, rho0, sc);
+        if (!st.ok()) return set_err(st.code, st.msg);
+        HIP_TRY(hipMemcpy(F.rho, rho0.data(), n * sizeof(double), hipMemcpyHostToDevice));
+    }
+    HIP_TRY(hipMemset(F.phi, 0, n * sizeof(double)));
+    memset(res, 0, sizeof *res);
+    double t_sq = 0.0, t_poisson = 0.0;
+    int nhist = 0, head = 0;
+    const int ext0[3] = {S.ext0[0], S.ext0[1], S.ext0[2]}, extn[3] = {S.extn[0], S.extn[1], S.extn[2]};
+    for (int it = 1; it <= maxit; it++) {
+        // V_eff pipeline
+        HIP_TRY(launch_xc(F.rho, n, F.eps, F.vxc, c->stream));
+        HIP_TRY(launch_axpbypgz(1.0, F.rho, 0.0, F.veff, 1.0, F.b, n, c->stream));   // rho + b
+        const auto tp = std::chrono::steady_clock::now();
+        int pits = 0;
+        if ((rc = scf_poisson(c, F.veff, tolp, 20000, &pits))) return rc;
+        t_poisson += std::chrono::duration<double>(std::chrono::steady_clock::now() - tp).count();
+        HIP_TRY(launch_axpbypgz(1.0, F.phi, 0.0, F.veff, 1.0, F.vxc, n, c->stream));  // phi + V_xc
+        HIP_TRY(launch_veff_to_diag(F.veff, S.cdiag, ext0, extn, G, c->diag64, c->diag32, c->stream));
+        // SQ pass and density
+        const auto ts = std::chrono::steady_clock::now();
+        if ((rc = sq_run(c, opt.nonlocal_mode, opt.precision, 0, n))) return rc;
+        double lf, eb, mts;
+        if ((rc = density_device(c, 0.0, 0.0, &lf, F.rho_out, &eb, &mts))) return rc;
+        t_sq += std::chrono::duration<double>(std::chrono::steady_clock::now() - ts).count();
+        // energies from the input density
+        double exc, evxc, pb, prho, rr, oo;
+        if ((rc = scf_dot(c, F.rho, F.eps, 6, &exc)) || (rc = scf_dot(c, F.rho, F.vxc, 7, &evxc)) ||
+            (rc = scf_dot(c, F.phi, F.b, 8, &pb)) || (rc = scf_dot(c, F.phi, F.rho, 9, &prho)))
+            return rc;
+        // residual f = rho_out - rho_in
+        HIP_TRY(launch_axpbypgz(1.0, F.rho_out, 0.0, F.f, -1.0, F.rho, n, c->stream));
+        if ((rc = scf_dot(c, F.f, F.f, 10, &rr)) || (rc = scf_dot(c, F.rho_out, F.rho_out, 11, &oo))) return rc;
+        res->iterations = it;
+        res->residual = std::sqrt(rr / oo);
+        res->lambda_f = lf;
+        res->e_band = eb;
+        res->minus_ts = mts;
+        res->e_xc = exc * dV;
+        res->e_vxc = evxc * dV;
+        res->e_el = 0.5 * (pb - prho) * dV;
+        res->e_self = F.eself;
+        res->e_total = eb + res->e_xc - res->e_vxc + res->e_el - F.eself + mts;
+        res->poisson_iterations = pits;
+        if (opt.verbose)
+            printf("scf %3d  residual %.3e  E %.12f  lambda_f %.10f  poisson %d\n", it, res->residual, res->e_total, lf, pits);
+        if (res->residual < tol) { res->converged = 1; break; }
+        // Periodic Pulay: history of iterate and residual differences
+        if (it > 1) {
+            double *dx = F.dX + (size_t)head * n, *df = F.dF + (size_t)head * n;
+            HIP_TRY(launch_axpbypgz(1.0, F.rho, 0.0, dx, -1.0, F.x_prev, n, c->stream));
+            HIP_TRY(launch_axpbypgz(1.0, F.f, 0.0, df, -1.0, F.f_prev, n, c->stream));
+            head = (head + 1) % m;
+            nhist = std::min(nhist + 1, m);
+        }
+        HIP_TRY(hipMemcpyAsync(F.x_prev, F.rho, n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+        HIP_TRY(hipMemcpyAsync(F.f_prev, F.f, n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+        int mu = 0;
+        if (it % pf == 0 && nhist > 0) {
+            std::vector<double> A((size_t)nhist * nhist), rhs(nhist);
+            for (int a = 0; a < nhist; a++) {
+                for (int b2 = 0; b2 <= a; b2++) {
+                    double v;
+                    if ((rc = scf_dot(c, F.dF + (size_t)a * n, F.dF + (size_t)b2 * n, 12, &v))) return rc;
+                    A[a * nhist + b2] = A[b2 * nhist + a] = v;
+                }
+                if ((rc = scf_dot(c, F.dF + (size_t)a * n, F.f, 12, &rhs[a]))) return rc;
+            }
+            if (small_solve(A, rhs, nhist)) {
+                HIP_TRY(hipMemcpy(F.g, rhs.data(), nhist * sizeof(double), hipMemcpyHostToDevice));
+                mu = nhist;
+            }
+        }
+        HIP_TRY(launch_pulay_update(F.rho, F.f, F.dX, F.dF, F.g, mu, beta, n, F.rho_out, c->stream));
+        std::swap(F.rho, F.rho_out);   // rho_out now holds the mixed density; keep rho as the input
+    }
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    res->wall_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    res->sq_s = t_sq;
+    res->poisson_s = t_poisson;
+    return SQ_OK;
+}
+
+int sq_get_cell_field(sq_ctx *c, int which, double *out)
+{
+    if (!c || !out) return set_err(SQ_ERR_ARG, "null argument");
+    auto &F = c->scf;
+    if (which == SQ_FIELD_B) {
+        int G[3], rc;
+        if ((rc = cell_dims(c, G))) return rc;
+        if (!F.have_b) {
+            Status st = build_pseudocharge(c->sys, F.host_b, F.eself, F.b_scale);
+            if (!st.ok()) return set_err(st.code, st.msg);
+            F.have_b = true;
+        }
+        memcpy(out, F.host_b.data(), F.host_b.size() * sizeof(double));
+        return SQ_OK;
+    }
+    if (which == SQ_FIELD_RHO_ATOMIC) {
+        int G[3], rc;
+        if ((rc = cell_dims(c, G))) return rc;
+        std::vector<double> r;
+        double sc;
+        Status st = build_atomic_density(c->sys, r, sc);
+        if (!st.ok()) return set_err(st.code, st.msg);
+        memcpy(out, r.data(), r.size() * sizeof(double));
+        return SQ_OK;
+    }
+    if (!F.rho) return set_err(SQ_ERR_STATE, "no SCF state yet");
+    const double *src = which == SQ_FIELD_RHO ? F.rho : which == SQ_FIELD_RHO_OUT ? F.rho_out : which == SQ_FIELD_PHI ? F.phi
+                      : which == SQ_FIELD_VXC ? F.vxc : which == SQ_FIELD_VEFF ? F.veff : nullptr;
+    if (!src) return set_err(SQ_ERR_ARG, "unknown field");
+    HIP_TRY(hipSetDevice(c->device));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    HIP_TRY(hipMemcpy(out, src, F.ng * sizeof(double), hipMemcpyDeviceToHost));
+    return SQ_OK;
+}
+
+int sq_set_veff_cell(sq_ctx *c, const double *veff)
+{
+    if (!c || !veff) return set_err(SQ_ERR_ARG, "null argument");
+    if (!c->stream) return set_err(SQ_ERR_STATE, "host-only context (created with device < 0)");
+    const System &S = c->sys;
+    int G[3] = {S.npts[0] - 1, S.npts[1] - 1, S.npts[2] - 1};
+    const long ng = (long)G[0] * G[1] * G[2];
+    HIP_TRY(hipSetDevice(c->device));
+    double *d = nullptr;
+    HIP_TRY(hipMalloc((void **)&d, ng * sizeof(double)));
+    const int ext0[3] = {S.ext0[0], S.ext0[1], S.ext0[2]}, extn[3] = {S.extn[0], S.extn[1], S.extn[2]};
+    int rc = SQ_OK;
+    if (hipMemcpy(d, veff, ng * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+        launch_veff_to_diag(d, S.cdiag, ext0, extn, G, c->diag64, c->diag32, c->stream) != hipSuccess ||
+        hipStreamSynchronize(c->stream) != hipSuccess)
+        rc = set_err(SQ_ERR_DEVICE, "V_eff upload failed");
+    (void)hipFree(d);
+    c->have_run = false;
+    return rc;
+}
+
+int sq_xc(sq_ctx *c, const double *rho, long n, double *eps, double *vxc)
+{
+    if (!c || !rho || !eps || !vxc || n <= 0) return set_err(SQ_ERR_ARG, "bad argument");
+    if (!c->stream) return set_err(SQ_ERR_STATE, "host-only context (created with device < 0)");
+    HIP_TRY(hipSetDevice(c->device));
+    double *d = nullptr;
+    HIP_TRY(hipMalloc((void **)&d, 3 * n * sizeof(double)));
+    int rc = SQ_OK;
+    if (hipMemcpy(d, rho, n * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+        launch_xc(d, n, d + n, d + 2 * n, c->stream) != hipSuccess || hipStreamSynchronize(c->stream) != hipSuccess ||
+        hipMemcpy(eps, d + n, n * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(vxc, d + 2 * n, n * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
+        rc = set_err(SQ_ERR_DEVICE, "xc device step failed");
+    (void)hipFree(d);
+    return rc;
+}
+
+int sq_poisson(sq_ctx *c, const double *f, double tol, double *phi, int *iters)
+{
+    if (!c || !f || !phi) return set_err(SQ_ERR_ARG, "null argument");
+    if (!c->stream) return set_err(SQ_ERR_STATE, "host-only context (created with device < 0)");
+    int G[3], rc;
+    if ((rc = cell_dims(c, G))) return rc;
+    HIP_TRY(hipSetDevice(c->device));
+    if ((rc = scf_alloc(c, std::max(1, c->scf.hist)))) return rc;
+    auto &F = c->scf;
+    HIP_TRY(hipMemcpy(F.eps, f, F.ng * sizeof(double), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemset(F.phi, 0, F.ng * sizeof(double)));
+    if ((rc = scf_poisson(c, F.eps, tol > 0 ? tol : c->sys.tol_poisson, 20000, iters))) return rc;
+    HIP_TRY(hipMemcpy(phi, F.phi, F.ng * sizeof(double), hipMemcpyDeviceToHost));
     return SQ_OK;
 }
 

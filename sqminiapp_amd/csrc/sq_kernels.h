@@ -65,4 +65,21 @@ hipError_t launch_density_energy(const double *lam, const double *w, long n_node
                                  double *out2, hipStream_t st);
 hipError_t launch_lam_range(const double *lmin, const double *lmax, long n, double *out, hipStream_t st);
 
+// ---- V_eff pipeline of the SCF (sq_scf.hip); fields on the periodic cell G
+struct StencilCoeffs {
+    double w[3][11];   // -(1/2) Lap_h weights per axis, w[d][0] the centre
+};
+hipError_t launch_xc(const double *rho, long n, double *eps, double *vxc, hipStream_t st);
+hipError_t launch_lap_periodic(const double *x, double *y, const int G[3], int order, const StencilCoeffs &c, hipStream_t st);
+hipError_t launch_dot(const double *a, const double *b, long n, double scale, double *partial, int nblocks, double *out,
+                      hipStream_t st);
+hipError_t launch_axpbypgz(double alpha, const double *x, double beta, double *y, double gamma, const double *z, long n,
+                           hipStream_t st);
+hipError_t launch_cg_xr(double *x, double *r, const double *p, const double *Ap, const double *s, long n, hipStream_t st);
+hipError_t launch_cg_p(double *p, const double *r, double *s, long n, hipStream_t st);
+hipError_t launch_pulay_update(const double *x, const double *f, const double *dX, const double *dF, const double *g, int m,
+                               double beta, long n, double *xn, hipStream_t st);
+hipError_t launch_veff_to_diag(const double *veff, double cdiag, const int ext0[3], const int extn[3], const int G[3],
+                               double *d64, float *d32, hipStream_t st);
+
 }  // namespace sqamd

@@ -161,6 +161,13 @@ Status parse_inpt(const std::string &path, System &S)
         else if (tok == "RADIUS_TRUNC:") ok = need(S.rcut);
         else if (tok == "LANCZOS_DEGREE:") ok = need(S.nqp);
         else if (tok == "TOL_PSEUDOCHARGE:") ok = need(S.tol_psc);
+        else if (tok == "MIXING_PARAMETER:") ok = need(S.mix_beta);
+        else if (tok == "MIXING_HISTORY:") ok = need(S.mix_history);
+        else if (tok == "PULAY_FREQUENCY:") ok = need(S.pulay_frequency);
+        else if (tok == "TOL_SCF:") ok = need(S.tol_scf);
+        else if (tok == "TOL_POISSON:") ok = need(S.tol_poisson);
+        else if (tok == "MAXIT_SCF:") ok = need(S.maxit_scf);
+        else if (tok == "EXCHANGE_CORRELATION:") ok = need(S.xc);
         else if (tok == "NTYPE:") ok = need(S.ntype) && S.ntype >= 1 && S.ntype <= kMaxType;
         else if (tok == "PSEUDOPOTENTIAL_LOCAL:") {
             ok = S.ntype > 0;
@@ -331,6 +338,7 @@ Status fit_pseudopotential(const System &S, Pseudopotential &P)
         return notaknot_slopes(P.r.data(), y.data(), P.size, s.yd);
     };
     if (!fit(P.s_vloc, P.vloc)) return fail(3, "singular spline system (Vloc)");
+    if (!fit(P.s_uu, P.uu)) return fail(3, "singular spline system (valence density)");
     for (int l = 0; l <= P.lmax; l++) {
         std::vector<double> udv(P.size);
         for (int q = 0; q < P.size; q++) udv[q] = P.U[l][q] * (P.V[l][q] - P.vloc[q]);
@@ -583,6 +591,228 @@ void build_coverage(System &S)
 }
 
 }  // namespace
+
+// ------------------------------------------------------------ SCF fields
+// The reference builds these pieces but never assembles them (its SCF is
+// declared only): the pseudocharge -(1/4pi) Lap_h V_loc of
+// Calculate_PseudochargeCutoff (lsdft_initObjs.cc:628-817) placed on each
+// atom image's local-overlap box (GetInfluencingAtoms_Local, 1409-1665), and
+// the isolated-atom valence density of the .pot file (lsdft_readfiles.cc:
+// 1026-1040).  The grid is the periodic cell of distinct points: the images
+// repeat every 2R = (N-1) h, so plane N-1 repeats plane 0.  Threads split the
+// cell into z-slabs and each walks every image in the same order, so every
+// point sums its images in one fixed order whatever the thread count.
+
+namespace {
+
+bool vloc_at(const System &S, int at, double r, double *out)
+{
+    const Pseudopotential &P = S.psd[at];
+    if (r > P.r[P.size - 1]) { *out = -(double)S.noe[at] / r; return true; }  // Coulomb tail
+    return P.s_vloc.eval(r, out);
+}
+
+struct CellImage {
+    int at;
+    double X[3];
+    int lo[3], hi[3];   // box, unwrapped global indices
+    bool self;          // image 0 of an atom
+};
+
+// Atom images in the loop order atom type, atom, shift (x, y, z), whose box
+// meets the cell or that are an atom's own (unshifted) image.
+std::vector<CellImage> cell_images(const System &S, const int G[3])
+{
+    std::vector<CellImage> out;
+    for (int at = 0; at < S.ntype; at++) {
+        int Imax[3], off[3];
+        for (int d = 0; d < 3; d++) {
+            Imax[d] = (int)std::ceil(S.cutoff[at][d] / S.range[d]);
+            off[d] = (int)std::ceil(S.cutoff[at][d] / S.h[d] + 0.5);
+        }
+        for (int p = 0; p < S.nAtoms; p++) {
+            if (S.atomtype[p] != at) continue;
+            const double *A = &S.atompos[3 * p];
+            for (int PP = -Imax[0]; PP <= Imax[0]; PP++)
+                for (int QQ = -Imax[1]; QQ <= Imax[1]; QQ++)
+                    for (int RR = -Imax[2]; RR <= Imax[2]; RR++) {
+                        CellImage im;
+                        im.at = at;
+                        im.X[0] = A[0] + PP * 2.0 * S.range[0];
+                        im.X[1] = A[1] + QQ * 2.0 * S.range[1];
+                        im.X[2] = A[2] + RR * 2.0 * S.range[2];
+                        bool meets = true;
+                        for (int d = 0; d < 3; d++) {
+                            const int c = nearest_index((im.X[d] + S.range[d]) / S.h[d]);
+                            im.lo[d] = c - off[d];
+                            im.hi[d] = c + off[d];
+                            meets = meets && im.lo[d] <= G[d] - 1 && im.hi[d] >= 0;
+                        }
+                        im.self = PP == 0 && QQ == 0 && RR == 0;
+                        if (meets || im.self) out.push_back(im);
+                    }
+        }
+    }
+    return out;
+}
+
+// b_J = -(1/4pi) Lap_h V_J on the sub-box lo..hi (inclusive) of an image, in
+// the expression and order of lsdft_initObjs.cc:725-735; V sampled on the
+// sub-box widened by the stencil half-width.  bj (and vj, if given) are the
+// sub-box, i fastest.
+bool image_pseudocharge(const System &S, const CellImage &im, const int lo[3], const int hi[3], double sc[3][kMaxOrder + 1],
+                        std::vector<double> &bj, std::vector<double> *vj)
+{
+    const int o = S.order;
+    int n[3], e[3];
+    for (int d = 0; d < 3; d++) { n[d] = hi[d] - lo[d] + 1; e[d] = n[d] + 2 * o; }
+    std::vector<double> V((size_t)e[0] * e[1] * e[2]);
+    auto VV = [&](int k, int j, int i) -> double & { return V[((size_t)k * e[1] + j) * e[0] + i]; };
+    for (int k = 0; k < e[2]; k++)
+        for (int j = 0; j < e[1]; j++)
+            for (int i = 0; i < e[0]; i++) {
+                double x = S.h[0] * (lo[0] - o + i) - S.range[0] - im.X[0];
+                double y = S.h[1] * (lo[1] - o + j) - S.range[1] - im.X[1];
+                double z = S.h[2] * (lo[2] - o + k) - S.range[2] - im.X[2];
+                double r = std::sqrt(x * x + y * y + z * z);
+                if (!vloc_at(S, im.at, r, &VV(k, j, i))) return false;
+            }
+    bj.assign((size_t)n[0] * n[1] * n[2], 0.0);
+    if (vj) vj->assign(bj.size(), 0.0);
+    for (int k = o; k < n[2] + o; k++)
+        for (int j = o; j < n[1] + o; j++)
+            for (int i = o; i < n[0] + o; i++) {
+                double v = VV(k, j, i) * (sc[0][0] + sc[1][0] + sc[2][0]);
+                for (int a = 1; a <= o; a++)
+                    v += (VV(k, j, i - a) + VV(k, j, i + a)) * sc[0][a] + (VV(k, j - a, i) + VV(k, j + a, i)) * sc[1][a] +
+                         (VV(k - a, j, i) + VV(k + a, j, i)) * sc[2][a];
+                const size_t q = ((size_t)(k - o) * n[1] + (j - o)) * n[0] + (i - o);
+                bj[q] = v;
+                if (vj) (*vj)[q] = VV(k, j, i);
+            }
+    return true;
+}
+
+unsigned setup_threads() { return std::max(1u, std::min(16u, std::thread::hardware_concurrency())); }
+
+}  // namespace
+
+Status build_pseudocharge(const System &S, std::vector<double> &b, double &eself, double &scale)
+{
+    const int G[3] = {S.npts[0] - 1, S.npts[1] - 1, S.npts[2] - 1};
+    for (int d = 0; d < 3; d++)
+        if (G[d] < 1) return fail(4, "the periodic cell needs at least two grid points per direction");
+    const size_t ng = (size_t)G[0] * G[1] * G[2];
+    b.assign(ng, 0.0);
+    double sc[3][kMaxOrder + 1];
+    for (int d = 0; d < 3; d++)
+        for (int p = 0; p <= S.order; p++) sc[d][p] = S.coeff[d][p] / (2.0 * M_PI);
+    const std::vector<CellImage> imgs = cell_images(S, G);
+    const unsigned nt = setup_threads();
+    std::atomic<int> bad{0};
+    // slabs of z-planes: each thread adds every image to its own planes, in image order
+    auto slab_worker = [&](unsigned t) {
+        const int z0 = (int)((long)G[2] * t / nt), z1 = (int)((long)G[2] * (t + 1) / nt) - 1;
+        if (z0 > z1) return;
+        std::vector<double> bj;
+        for (const CellImage &im : imgs) {
+            int lo[3], hi[3];
+            bool empty = false;
+            for (int d = 0; d < 3; d++) {
+                lo[d] = std::max(im.lo[d], 0);
+                hi[d] = std::min(im.hi[d], G[d] - 1);
+            }
+            lo[2] = std::max(lo[2], z0);
+            hi[2] = std::min(hi[2], z1);
+            for (int d = 0; d < 3; d++) empty = empty || lo[d] > hi[d];
+            if (empty) continue;
+            if (!image_pseudocharge(S, im, lo, hi, sc, bj, nullptr)) { bad = 3; return; }
+            const int nx = hi[0] - lo[0] + 1, ny = hi[1] - lo[1] + 1;
+            for (int k = lo[2]; k <= hi[2]; k++)
+                for (int j = lo[1]; j <= hi[1]; j++)
+                    for (int i = lo[0]; i <= hi[0]; i++)
+                        b[((size_t)k * G[1] + j) * G[0] + i] += bj[((size_t)(k - lo[2]) * ny + (j - lo[1])) * nx + (i - lo[0])];
+        }
+    };
+    // self energy of each atom over its whole box, summed afterwards in atom order
+    std::vector<const CellImage *> selves;
+    for (const CellImage &im : imgs)
+        if (im.self) selves.push_back(&im);
+    std::vector<std::vector<double>> prod(selves.size());
+    std::atomic<size_t> next{0};
+    auto self_worker = [&]() {
+        std::vector<double> bj, vj;
+        for (size_t a; (a = next++) < selves.size();) {
+            if (!image_pseudocharge(S, *selves[a], selves[a]->lo, selves[a]->hi, sc, bj, &vj)) { bad = 3; return; }
+            prod[a].resize(bj.size());
+            for (size_t q = 0; q < bj.size(); q++) prod[a][q] = bj[q] * vj[q];
+        }
+    };
+    std::vector<std::thread> pool;
+    for (unsigned t = 0; t < nt; t++) pool.emplace_back(slab_worker, t);
+    for (auto &t : pool) t.join();
+    pool.clear();
+    for (unsigned t = 0; t < nt; t++) pool.emplace_back(self_worker);
+    for (auto &t : pool) t.join();
+    if (bad) return fail(3, "pseudocharge: V_loc spline evaluation failed");
+    double es = 0.0;
+    for (const auto &pr : prod)
+        for (double v : pr) es += v;
+    double tot = 0.0;
+    for (size_t q = 0; q < ng; q++) tot += b[q];
+    tot *= S.dV;
+    const double f = -(double)S.noetot / tot;
+    for (size_t q = 0; q < ng; q++) b[q] *= f;
+    scale = f;
+    eself = 0.5 * es * S.dV * f;
+    return {};
+}
+
+Status build_atomic_density(const System &S, std::vector<double> &rho, double &scale)
+{
+    const int G[3] = {S.npts[0] - 1, S.npts[1] - 1, S.npts[2] - 1};
+    for (int d = 0; d < 3; d++)
+        if (G[d] < 1) return fail(4, "the periodic cell needs at least two grid points per direction");
+    const size_t ng = (size_t)G[0] * G[1] * G[2];
+    rho.assign(ng, 0.0);
+    const std::vector<CellImage> imgs = cell_images(S, G);
+    const unsigned nt = setup_threads();
+    std::atomic<int> bad{0};
+    auto worker = [&](unsigned t) {
+        const int z0 = (int)((long)G[2] * t / nt), z1 = (int)((long)G[2] * (t + 1) / nt) - 1;
+        for (const CellImage &im : imgs) {
+            const Pseudopotential &P = S.psd[im.at];
+            int lo[3], hi[3];
+            for (int d = 0; d < 3; d++) {
+                lo[d] = std::max(im.lo[d], 0);
+                hi[d] = std::min(im.hi[d], G[d] - 1);
+            }
+            lo[2] = std::max(lo[2], z0);
+            hi[2] = std::min(hi[2], z1);
+            for (int k = lo[2]; k <= hi[2]; k++)
+                for (int j = lo[1]; j <= hi[1]; j++)
+                    for (int i = lo[0]; i <= hi[0]; i++) {
+                        double x = S.h[0] * i - S.range[0] - im.X[0];
+                        double y = S.h[1] * j - S.range[1] - im.X[1];
+                        double z = S.h[2] * k - S.range[2] - im.X[2];
+                        double r = std::sqrt(x * x + y * y + z * z), v = 0.0;
+                        if (r <= P.r[P.size - 1] && !P.s_uu.eval(r, &v)) { bad = 3; return; }
+                        rho[((size_t)k * G[1] + j) * G[0] + i] += v;
+                    }
+        }
+    };
+    std::vector<std::thread> pool;
+    for (unsigned t = 0; t < nt; t++) pool.emplace_back(worker, t);
+    for (auto &t : pool) t.join();
+    if (bad) return fail(3, "atomic density: spline evaluation failed");
+    double tot = 0.0;
+    for (size_t q = 0; q < ng; q++) tot += rho[q];
+    tot *= S.dV;
+    const double f = (double)S.noetot / tot;
+    for (size_t q = 0; q < ng; q++) rho[q] *= f;
+    scale = f;
+    return {};
+}
 
 Status build_system(const std::string &dir, const std::string &name, const SetupOptions &opt, System &S)
 {

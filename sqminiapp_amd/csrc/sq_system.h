@@ -33,7 +33,7 @@ struct Pseudopotential {
     std::vector<double> r, uu, vloc;
     std::vector<double> V[4], U[4];
     double rc[4] = {0, 0, 0, 0};
-    RadialSpline s_vloc, s_udv[4], s_u[4];
+    RadialSpline s_vloc, s_udv[4], s_u[4], s_uu;  // s_uu: isolated-atom valence density
     double gamma[4] = {0, 0, 0, 0};
     // (l,m) channel table of the nonlocal part: channel c -> l, m; l != lloc.
     std::vector<int> ch_l, ch_m;
@@ -57,6 +57,10 @@ struct System {
     double range[3] = {0, 0, 0};
     int order = 6, bc = 1, nqp = 50, ntype = -1;
     double beta = 1000, rcut = 12.0, tol_psc = 1e-6;
+    // SCF keywords (Read_Parameters defaults, lsdft_readfiles.cc:62-102, 219-240)
+    double mix_beta = 0.3, tol_scf = 1e-6, tol_poisson = 1e-8;
+    int mix_history = 7, pulay_frequency = 1, maxit_scf = 100;
+    std::string xc = "LDA";
     int localPsd[kMaxType] = {0};
     std::string psdFile[kMaxType];
     // ---- derived grid quantities
@@ -106,6 +110,14 @@ struct SetupOptions {
 
 // Reads the deck and builds every host table of the path.
 Status build_system(const std::string &dir, const std::string &name, const SetupOptions &opt, System &sys);
+
+// SCF fields on the periodic cell: the (N0-1) x (N1-1) x (N2-1) distinct grid
+// points, i fastest.  b: pseudocharge -(1/4pi) Lap_h V_loc of every atom image
+// on its local-overlap box, scaled to -N_e (scale returned), with the self
+// energy (1/2) sum_J int b_J V_J; rho: superposed isolated-atom valence
+// densities scaled to N_e.
+Status build_pseudocharge(const System &S, std::vector<double> &b, double &eself, double &scale);
+Status build_atomic_density(const System &S, std::vector<double> &rho, double &scale);
 
 // Pieces exposed for the reference-named host entry points.
 double real_spherical_harmonic(double x, double y, double z, int l, int m, bool *bad);

@@ -38,7 +38,11 @@ class Supercell:
     h_target: float = 0.5
     rcut: float = 7.0
     nqp: int = 80
-    beta: float = 1000.0
+    # kT = 10 mHa: with 80 Gauss nodes per node the quadrature resolves the
+    # Fermi function only to ~ (spectral width)/80, so at kT = 1 mHa the SQ
+    # density jumps as the Fermi level moves and the SCF fixed point is not
+    # reached (measured on the 16-atom cell); kT = 10 mHa converges in ~25 steps
+    beta: float = 100.0
     vacancy_cluster: int = 0          # remove the atom nearest the centre and its n-1 nearest neighbours
     extra: dict = field(default_factory=dict)
 

@@ -63,7 +63,7 @@ def test_c16_full_pass_invariants(c16):
 
 def test_c16_fp32_error_against_fp64(c16):
     """FP32 recurrence, FP64 QL/Fermi/sums, against the FP64 form on every node.
-    At BETA 1000 (kT = 1 mHa) the occupation is a near step, so a node whose
+    At BETA 100 (kT = 10 mHa) the occupation is still steep, so a node whose
     Gauss nodes straddle lambda_f moves its density by a few percent; the
     integrated quantities stay within the bounds below."""
     ctx, _, _ = c16

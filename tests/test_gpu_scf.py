@@ -1,0 +1,101 @@
+"""The SCF loop on the GPU (include/sq_amd.h, SCF section) against the oracle:
+each V_eff-pipeline piece on its own (LDA, Poisson, V_eff on the cell), then
+whole SCF runs: a small-cube deck against the oracle SCF with the same mixing,
+and the Mg1 deck in the FP64 and FP32 forms.
+
+Tolerances: LDA 1e-13 relative (same formulas, device libm); Poisson 1e-8
+relative (conjugate gradients to 1e-12 against the FFT solve); Lanczos nodes
+with a V_eff field 1e-12; converged SCF energies 1e-8 Ha (both stop at a
+density residual of 1e-9); FP32 against FP64 1e-5 Ha/atom.
+"""
+import numpy as np
+import pytest
+
+from conftest import MG1, rel_err, write_variant
+from oracle import oracle as O
+from oracle import scf as OS
+
+import sqminiapp_amd as sq
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def mg1_cell_oracle():
+    S = O.System(MG1, "Mg1")
+    S.set_phantom(False)
+    S.set_cell_period(True)
+    S.build_nonlocal(S.cell())
+    return S
+
+
+def test_xc_matches_oracle():
+    rng = np.random.default_rng(1)
+    rho = np.concatenate([10.0 ** rng.uniform(-8, 1, 4000), [0.0, -1e-4, 1e-300]])
+    with sq.Context(MG1, "Mg1", cell=True) as c:
+        eps, v = c.xc(rho)
+    eo, vo = OS.xc_lda_pw(rho)
+    assert rel_err(eps, eo) < 1e-13 and rel_err(v, vo) < 1e-13
+    assert np.all(eps[-3:-1] == 0) and np.all(v[-3:-1] == 0)
+
+
+def test_poisson_cg_matches_fft(mg1_cell_oracle):
+    I = mg1_cell_oracle.info()
+    with sq.Context(MG1, "Mg1", cell=True) as c:
+        f = c.cell_field("rho_atomic") + c.cell_field("b")
+        phi, its = c.poisson(f, tol=1e-12)
+    coeffs = [list(I.coeffs[d]) for d in range(3)]
+    ref = OS.poisson_fft(coeffs, I.order, f)
+    print("Poisson CG iterations", its)
+    assert its > 0 and rel_err(phi, ref) < 1e-8
+    assert abs(phi.mean()) < 1e-12 * np.abs(phi).max()
+
+
+def test_veff_on_the_cell_matches_oracle(mg1_cell_oracle):
+    I = mg1_cell_oracle.info()
+    G = (I.npts[2] - 1, I.npts[1] - 1, I.npts[0] - 1)
+    g = np.indices(G).astype(float)
+    field = 0.2 + 0.1 * np.sin(2 * np.pi * g[2] / G[2]) * np.cos(2 * np.pi * g[1] / G[1]) + 0.05 * np.cos(2 * np.pi * g[0] / G[0])
+    ids = list(range(0, 2720, 97))
+    lo, wo, _, _ = mg1_cell_oracle.nodes_and_weights(mode=1, veff=field.ravel(), node_ids=ids, nthreads=4)
+    with sq.Context(MG1, "Mg1", cell=True) as c:
+        c.set_veff_cell(field)
+        c.run(sq.SQ_NL_KB, sq.SQ_FP64)
+        lam, w, _, _ = c.results()
+    assert rel_err(np.sort(lam[ids], 1), np.sort(lo, 1)) < 1e-12
+
+
+def test_scf_small_cube_matches_oracle_scf(tmp_path):
+    d, n = write_variant(str(tmp_path), "T", rcut=1.3)
+    S = O.System(d, n)
+    S.set_phantom(False)
+    S.set_cell_period(True)
+    S.build_nonlocal(S.cell())
+    ref = OS.scf(S, mode=1, tol=1e-9, maxit=80, beta=0.3, history=7, frequency=2)
+    assert ref["converged"]
+    with sq.Context(d, n, cell=True) as c:
+        r = c.scf(sq.SQ_NL_KB, sq.SQ_FP64, tol=1e-9, max_iter=80, mixing=0.3, history=7, pulay_frequency=2,
+                  tol_poisson=1e-12)
+        rho = c.cell_field("rho")
+    print("small cube: GPU", r["iterations"], "iterations, oracle", ref["iterations"])
+    assert r["converged"]
+    assert abs(r["iterations"] - ref["iterations"]) <= 3
+    for k in ("e_total", "e_band", "e_xc", "e_el", "lambda_f"):
+        assert r[k] == pytest.approx(ref[k], abs=1e-8), k
+    assert r["e_self"] == pytest.approx(ref["e_self"], rel=1e-12)
+    assert rel_err(rho, ref["rho_in"]) < 1e-7
+
+
+def test_scf_mg1_fp64_and_fp32_forms():
+    """The FP32 recurrence leaves ~1e-6 relative noise in rho, so the FP32 form
+    stops at a density residual of 1e-5; its energy is judged against FP64."""
+    with sq.Context(MG1, "Mg1", cell=True) as c:
+        r64 = c.scf(sq.SQ_NL_KB, sq.SQ_FP64, tol=1e-7, max_iter=200, mixing=0.3)
+        r32 = c.scf(sq.SQ_NL_KB, sq.SQ_FP32, tol=1e-5, max_iter=200, mixing=0.3)
+    natoms = 4
+    de = abs(r32["e_total"] - r64["e_total"]) / natoms
+    print("Mg1 SCF: FP64", r64["iterations"], "it", r64["wall_s"], "s E", r64["e_total"],
+          "| FP32", r32["iterations"], "it", r32["wall_s"], "s dE/atom", de)
+    assert r64["converged"] and r32["converged"]
+    assert de < 1e-5
+    assert abs(r32["lambda_f"] - r64["lambda_f"]) < 1e-5

@@ -1,0 +1,186 @@
+"""Pins the oracle's SCF pieces (oracle/scf.py, the pseudocharge and atomic
+density of oracle/sq_oracle.c) without the reference, which declares its SCF
+but does not build it:
+
+- LDA: Slater exchange in closed form, PW92 correlation against the
+  independent Perdew-Zunger 1981 fit of the same Ceperley-Alder data, and
+  v_xc = d(rho eps_xc)/d rho by central differences;
+- Poisson: the FFT solve with the stencil's exact symbol recovers a
+  manufactured potential whose right-hand side is built with the stencil in
+  real space (np.roll), and the reverse;
+- Periodic Pulay: converges on a linear fixed-point problem to the solution
+  numpy.linalg.solve gives, far faster than simple mixing;
+- pseudocharge / atomic density: a numpy + scipy restatement (scipy's
+  not-a-knot spline, vectorised boxes) gives the same fields; neutrality; the
+  host setup of the product is bit-identical to the oracle;
+- the oracle SCF on a small-cube deck converges to a self-consistent density.
+"""
+import numpy as np
+import pytest
+from scipy.interpolate import CubicSpline
+
+from conftest import MG1, write_variant
+from oracle import scf as OS
+
+import sqminiapp_amd as sq
+
+
+def test_exchange_closed_form_and_pw92_against_pz81():
+    rs = np.array([0.5, 0.8, 1.0, 2.0, 3.0, 5.0, 8.0, 10.0])
+    rho = 3.0 / (4.0 * np.pi * rs ** 3)
+    eps, _ = OS.xc_lda_pw(rho)
+    ex = -0.75 * (3.0 / np.pi) ** (1.0 / 3.0) * np.cbrt(rho)
+    assert np.allclose(ex, -0.4581652932831429 / rs, rtol=1e-14)   # -(3/4)(9/(4 pi^2))^(1/3)/rs
+    ec = eps - ex
+    pz = OS.pz81_eps_c(rs)
+    assert np.all(np.abs(ec - pz) / np.abs(pz) < 0.03), (ec, pz)
+    assert np.all(np.diff(ec) > 0)              # |eps_c| falls with rs
+
+
+def test_vxc_is_the_density_derivative():
+    rho = np.array([1e-4, 3e-3, 0.02, 0.1, 0.7, 4.0])
+    _, v = OS.xc_lda_pw(rho)
+    d = 1e-6
+    ep, _ = OS.xc_lda_pw(rho * (1 + d))
+    em, _ = OS.xc_lda_pw(rho * (1 - d))
+    fd = (rho * (1 + d) * ep - rho * (1 - d) * em) / (2 * d * rho)
+    assert np.allclose(v, fd, rtol=1e-8)
+    e0, v0 = OS.xc_lda_pw(np.array([0.0, -1e-3]))
+    assert np.all(e0 == 0) and np.all(v0 == 0)
+
+
+def _coeffs(S):
+    I = S.info()
+    return [list(I.coeffs[d]) for d in range(3)], I.order
+
+
+def test_poisson_fft_recovers_manufactured_potential(mg1_oracle):
+    coeffs, order = _coeffs(mg1_oracle)
+    rng = np.random.default_rng(4)
+    shape = (16, 17, 10)
+    phi = rng.standard_normal(shape)
+    phi -= phi.mean()
+    f = OS.apply_laplacian(coeffs, order, phi) / (2 * np.pi)      # -(1/4pi) Lap_h phi
+    assert abs(f.mean()) < 1e-12
+    phi2 = OS.poisson_fft(coeffs, order, f)
+    assert np.max(np.abs(phi2 - phi)) < 1e-10 * np.max(np.abs(phi))
+    # and the other way round: the stencil applied to the FFT solution returns f
+    g = rng.standard_normal(shape)
+    g -= g.mean()
+    sol = OS.poisson_fft(coeffs, order, g)
+    assert np.max(np.abs(OS.apply_laplacian(coeffs, order, sol) / (2 * np.pi) - g)) < 1e-10 * np.max(np.abs(g))
+
+
+def test_periodic_pulay_solves_a_linear_fixed_point():
+    rng = np.random.default_rng(9)
+    n = 40
+    Q, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    A = Q @ np.diag(np.linspace(-0.95, 0.95, n)) @ Q.T
+    c = rng.standard_normal(n)
+    xstar = np.linalg.solve(np.eye(n) - A, c)
+
+    def run(m, p, iters):
+        mix = OS.PeriodicPulay(0.2, m, p)
+        x = np.zeros(n)
+        for _ in range(iters):
+            x = mix.step(x, A @ x + c)
+        return np.linalg.norm(x - xstar) / np.linalg.norm(xstar)
+
+    assert run(20, 2, 80) < 1e-8
+    assert run(0, 10 ** 9, 80) > 1e-3          # simple mixing alone is far slower
+
+
+def _roundf(v):
+    """C roundf of the value converted to float: halves away from zero."""
+    f = np.float32(v)
+    return int(np.copysign(np.floor(np.abs(f) + np.float32(0.5)), f))
+
+
+def _numpy_fields(S):
+    """Pseudocharge and atomic density restated with numpy + scipy splines."""
+    I = S.info()
+    G = [I.npts[d] - 1 for d in range(3)]
+    o = I.order
+    rad = S.radial(0, 1)
+    r = rad["r"]
+    vl = CubicSpline(r, rad["vloc"], bc_type="not-a-knot")
+    sc = [np.array(I.coeffs[d][:o + 1]) / (2 * np.pi) for d in range(3)]
+    atoms = S.atom_positions()
+    h, R = np.array(I.delta[:]), np.array(I.range[:])
+    cut = np.array(I.cutoff[0][:])
+    off = np.ceil(cut / h + 0.5).astype(int)
+    imax = np.ceil(cut / R).astype(int)
+    b = np.zeros(G[::-1])
+    rho = np.zeros(G[::-1])
+    uu = S.valence_density(0)
+    ud = CubicSpline(r, uu, bc_type="not-a-knot")
+    for A in atoms:
+        for P in range(-imax[0], imax[0] + 1):
+            for Q in range(-imax[1], imax[1] + 1):
+                for Rr in range(-imax[2], imax[2] + 1):
+                    X = A + 2 * R * np.array([P, Q, Rr])
+                    c = np.array([_roundf((X[d] + R[d]) / h[d]) for d in range(3)])
+                    lo, hi = c - off, c + off
+                    clo, chi = np.maximum(lo, 0), np.minimum(hi, np.array(G) - 1)
+                    if np.any(clo > chi):
+                        continue
+                    ax = [np.arange(clo[d] - o, chi[d] + o + 1) * h[d] - R[d] - X[d] for d in range(3)]
+                    zz, yy, xx = np.meshgrid(ax[2], ax[1], ax[0], indexing="ij")
+                    V = vl(np.sqrt(xx ** 2 + yy ** 2 + zz ** 2))
+                    core = V[o:-o, o:-o, o:-o] * (sc[0][0] + sc[1][0] + sc[2][0])
+                    for a in range(1, o + 1):
+                        core = core + (V[o:-o, o:-o, o - a:V.shape[2] - o - a] + V[o:-o, o:-o, o + a:V.shape[2] - o + a]) * sc[0][a]
+                        core = core + (V[o:-o, o - a:V.shape[1] - o - a, o:-o] + V[o:-o, o + a:V.shape[1] - o + a, o:-o]) * sc[1][a]
+                        core = core + (V[o - a:V.shape[0] - o - a, o:-o, o:-o] + V[o + a:V.shape[0] - o + a, o:-o, o:-o]) * sc[2][a]
+                    sl = (slice(clo[2], chi[2] + 1), slice(clo[1], chi[1] + 1), slice(clo[0], chi[0] + 1))
+                    b[sl] += core
+                    rr = np.sqrt(xx[o:-o, o:-o, o:-o] ** 2 + yy[o:-o, o:-o, o:-o] ** 2 + zz[o:-o, o:-o, o:-o] ** 2)
+                    rho[sl] += ud(rr)
+    b *= -I.noetot / (b.sum() * I.dV)
+    rho *= I.noetot / (rho.sum() * I.dV)
+    return b, rho
+
+
+def test_pseudocharge_and_atomic_density_against_numpy_restatement():
+    from oracle import oracle as O
+    S = O.System(MG1, "Mg1")
+    S.set_phantom(False)
+    I = S.info()
+    b, eself, scale = S.pseudocharge()
+    rho, rscale = S.atomic_density()
+    assert b.sum() * I.dV == pytest.approx(-8.0, rel=1e-13)
+    assert rho.sum() * I.dV == pytest.approx(8.0, rel=1e-13)
+    assert abs(scale - 1.0) < 1e-5            # the cutoff keeps |int b + Z|/Z within TOL_PSEUDOCHARGE
+    assert eself > 0
+    bn, rn = _numpy_fields(S)
+    assert np.max(np.abs(b - bn.ravel())) < 1e-11 * np.max(np.abs(b))
+    assert np.max(np.abs(rho - rn.ravel())) < 1e-11 * np.max(np.abs(rho))
+
+
+def test_host_scf_fields_bit_identical_to_oracle(mg1_oracle):
+    from oracle import oracle as O
+    S = O.System(MG1, "Mg1")
+    S.set_phantom(False)
+    with sq.Context(MG1, "Mg1", device=-1, cell=True) as c:
+        assert tuple(c.info.rank_box) == (0, 0, 0, 10, 17, 16)
+        assert np.array_equal(c.cell_field("b").ravel(), S.pseudocharge()[0])
+        assert np.array_equal(c.cell_field("rho_atomic").ravel(), S.atomic_density()[0])
+    with sq.Context(MG1, "Mg1", device=-1) as c:      # the SCF needs the cell as the node box
+        with pytest.raises(sq.SQError) as e:
+            c.cell_field("b")
+        assert e.value.code == 9
+
+
+def test_oracle_scf_converges_on_a_small_cube_deck(tmp_path):
+    from oracle import oracle as O
+    d, n = write_variant(str(tmp_path), "T", rcut=1.3)
+    S = O.System(d, n)
+    S.set_phantom(False)
+    S.set_cell_period(True)
+    S.build_nonlocal(S.cell())
+    out = OS.scf(S, mode=1, tol=1e-6, maxit=40, beta=0.3, history=7, frequency=2)
+    assert out["converged"], out["residuals"][-5:]
+    I = S.info()
+    assert out["rho_out"].sum() * I.dV == pytest.approx(8.0, rel=1e-10)
+    # self-consistency: the last input density reproduces itself
+    assert np.linalg.norm(out["rho_out"] - out["rho_in"]) / np.linalg.norm(out["rho_out"]) < 1e-6
