@@ -41,6 +41,9 @@ struct SQ_LSDFT {
     double delVol = 0, Rcut = 0, Beta = 0;                  /* :171,173,191 */
     double coeffs_x[11] = {}, coeffs_y[11] = {}, coeffs_z[11] = {};  /* :182-184 */
     double Eband = 0, Entropy = 0;                          /* :158-159; Entropy holds -TS */
+    double Exc = 0, Eself = 0, Etot = 0;                    /* :152-154 and the SCF total energy */
+    int MAXITSCF = 0, SCFIterations = 0;                    /* :111; iterations of the last SCF */
+    double TOLSCF = 0, MixingParameter = 0;                 /* :156,200 (0: the deck's values) */
 
     /* DMDAGetCorners(da) of this rank (lsdft_scf.cc:49) */
     int xcor = 0, ycor = 0, zcor = 0, lxdim = 0, lydim = 0, lzdim = 0;
@@ -103,6 +106,14 @@ int CalculateDensity_lssgq(SQ_LSDFT *pLsdft);
 /* SystemEnergy_Calc_lssgq (declared lsdft.h:209): prints the band and
  * entropy energies that CalculateDensity_lssgq computed. */
 int SystemEnergy_Calc_lssgq(SQ_LSDFT *pLsdft);
+
+/* SCF_LSSGQ (declared lsdft.h:124) on the periodic cell: the context must
+ * have been initialized with rank_box = (0,0,0,N0-1,N1-1,N2-1).  Runs
+ * sq_scf with the deck's SCF keywords (or MAXITSCF / TOLSCF / MixingParameter
+ * when set), fills elecDensRho with the converged density and lambda_f,
+ * Eband, Entropy, Exc, Eself, Etot; returns 0 or the sq_status code
+ * (SQ_ERR_RANGE when the loop ends unconverged). */
+double SCF_LSSGQ(SQ_LSDFT *pLsdft);
 
 /* The reference's per-rank text files (lsdft_readfiles.cc:1106-1380):
  * <DenFilename>_<rank>.den holds "xcor ycor zcor lx ly lz" then "k j i rho"

@@ -166,6 +166,34 @@ int SystemEnergy_Calc_lssgq(SQ_LSDFT *p)
     return SQ_OK;
 }
 
+double SCF_LSSGQ(SQ_LSDFT *p)
+{
+    if (!ready(p)) return fail(SQ_ERR_STATE, "SCF_LSSGQ: Initialize first");
+    sq_scf_options o;
+    sq_default_scf_options(&o);
+    o.nonlocal_mode = p->nonlocal_mode;
+    o.precision = p->precision;
+    o.max_iter = p->MAXITSCF;
+    o.tol = p->TOLSCF;
+    o.mixing = p->MixingParameter;
+    sq_scf_result r;
+    int rc = sq_scf(p->ctx, &o, &r);
+    if (rc) return rc;
+    p->SCFIterations = r.iterations;
+    p->lambda_f = r.lambda_f;
+    p->Eband = r.e_band;
+    p->Entropy = r.minus_ts;
+    p->Exc = r.e_xc;
+    p->Eself = r.e_self;
+    p->Etot = r.e_total;
+    if ((rc = sq_get_cell_field(p->ctx, SQ_FIELD_RHO, p->elecDensRho.data()))) return rc;
+    // nodes, weights and extremes of the last SQ pass of the loop
+    if ((rc = sq_get_nodes_and_weights(p->ctx, p->lssgq_lambda.data(), p->lssgq_weights.data(), p->lambda_min.data(),
+                                       p->lambda_max.data())))
+        return rc;
+    return r.converged ? SQ_OK : fail(SQ_ERR_RANGE, "SCF_LSSGQ: not converged in MAXIT_SCF iterations");
+}
+
 int Print_ElecDens(SQ_LSDFT *p)
 {
     if (!ready(p)) return fail(SQ_ERR_STATE, "Print_ElecDens: Initialize first");

@@ -3,13 +3,15 @@
 //
 //   sq_main -name <dir>/<name> [-mode collapsed|kb] [-fp32] [-phantom [x y z]]
 //           [-rank_box xcor ycor zcor lx ly lz] [-rank r] [-nqp n] [-device d]
-//           [-out <DenFilename>] [-density] [-verify_files]
+//           [-out <DenFilename>] [-density] [-verify_files] [-scf]
 //
 // It prints the reference's progress lines and its two SQ timers, then writes
 // <DenFilename>_<rank>.lambdas (Print_MinMaxLambda) and, with -density, the
 // Fermi level, energies and <DenFilename>_<rank>.den (Print_ElecDens).  Any
 // failing step prints sq_last_error() and exits with status 1, where the
-// reference exits too.  -verify_files reads the written files back with
+// reference exits too.  -scf runs SCF_LSSGQ on the periodic cell instead of
+// one SQ pass (the node box becomes the cell) and prints its energies.
+// -verify_files reads the written files back with
 // Read_MinMaxLambda / Read_ElecDens and checks them against the results.
 #include <algorithm>
 #include <chrono>
@@ -39,7 +41,7 @@ int main(int argc, char **argv)
     SQ_LSDFT lsdft;
     sq_options opt;
     sq_default_options(&opt);
-    bool density = false, verify = false;
+    bool density = false, verify = false, scf = false;
     std::string out;
     for (int a = 1; a < argc; a++) {
         const std::string s = argv[a];
@@ -69,6 +71,7 @@ int main(int argc, char **argv)
         else if (s == "-out") { need(1); out = argv[++a]; }
         else if (s == "-density") density = true;
         else if (s == "-verify_files") verify = true;
+        else if (s == "-scf") scf = true;
         else { fprintf(stderr, "unknown option %s\n", s.c_str()); return 1; }
     }
     if (path.empty()) { fprintf(stderr, "usage: sq_main -name <dir>/<name> [options]\n"); return 1; }
@@ -77,6 +80,18 @@ int main(int argc, char **argv)
     const std::string name = slash == std::string::npos ? path : path.substr(slash + 1);
     lsdft.DenFilename = out.empty() ? "./" + name : out;
 
+    if (scf) {
+        // the SCF runs on the distinct points of the periodic cell
+        FILE *fp = fopen((dir + "/" + name + ".inpt").c_str(), "r");
+        char tok[256];
+        int N[3] = {0, 0, 0};
+        while (fp && fscanf(fp, "%255s", tok) == 1)
+            if (!strcmp(tok, "FD_GRID:") && fscanf(fp, "%d %d %d", &N[0], &N[1], &N[2]) == 3) break;
+        if (fp) fclose(fp);
+        if (N[0] < 2) { fprintf(stderr, "FD_GRID missing in the deck\n"); return 1; }
+        int box[6] = {0, 0, 0, N[0] - 1, N[1] - 1, N[2] - 1};
+        memcpy(opt.rank_box, box, sizeof box);
+    }
     const clock_t begin0 = clock();
     const double w1 = now();
     int rc = Initialize(&lsdft, dir.c_str(), name.c_str(), &opt);
@@ -91,7 +106,17 @@ int main(int argc, char **argv)
 
     const clock_t begin = clock();
     const double v1 = now();
-    if ((rc = (int)NodesAndWeights_LSSGQ(&lsdft))) die("NodesAndWeights_LSSGQ", rc);
+    if (scf) {
+        if ((rc = (int)SCF_LSSGQ(&lsdft))) die("SCF_LSSGQ", rc);
+        printf("SCF converged in %d iterations\n", lsdft.SCFIterations);
+        printf("Total energy     = %.12f (Ha)\n", lsdft.Etot);
+        printf("Total energy/atom= %.12f (Ha)\n", lsdft.Etot / lsdft.nAtoms);
+        printf("Exchange-corr.   = %.12f (Ha)\n", lsdft.Exc);
+        printf("Self energy      = %.12f (Ha)\n", lsdft.Eself);
+        SystemEnergy_Calc_lssgq(&lsdft);
+        density = false;   // rho of the SCF is already in elecDensRho
+        if ((rc = Print_ElecDens(&lsdft))) die("Print_ElecDens", rc);
+    } else if ((rc = (int)NodesAndWeights_LSSGQ(&lsdft))) die("NodesAndWeights_LSSGQ", rc);
     const double v2 = now();
     const clock_t end = clock();
     printf("rank=%d, Time spent in SQ=%lf \n", lsdft.rank, (double)(end - begin) / CLOCKS_PER_SEC);
