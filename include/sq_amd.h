@@ -154,6 +154,20 @@ int sq_tridiag_nodes_weights(sq_ctx *ctx, const double *diag, const double *subd
 int sq_density_fermi_energy(sq_ctx *ctx, double nelec, double beta, double *lambda_f, double *rho,
                             double *e_band, double *minus_ts);
 
+/* Per-node early exit of the batched Lanczos (the column-marching kernel): a
+ * node stops at n Lanczos steps once the lowest 8 Matsubara terms
+ * (2/beta) g(lambda_f + i (2m+1) pi / beta), g(z) = e1^T (z - T_n)^{-1} e1,
+ * of its Gauss rule change by less than tol between checks at n = min_steps,
+ * min_steps + every, ... (the Fermi quadrature is 1/2 + the sum of their real
+ * parts over all m), up to the deck's LANCZOS_DEGREE.  Its rule then has n
+ * nodes; the rest of its lambda/w row is padded with weight zero.  tol <= 0
+ * turns it off; beta <= 0 uses the deck's BETA.  The reference has no
+ * counterpart (it runs N_qp steps for every node, lsdft_scf.cc:137). */
+int sq_set_early_exit(sq_ctx *ctx, double tol, double lambda_f, double beta, int min_steps, int every);
+
+/* Gauss nodes per node of the last sq_run (N_qp unless it stopped early). */
+int sq_get_rule_sizes(sq_ctx *ctx, int *n);
+
 /* Wait for all device work of the context's device (hipDeviceSynchronize). */
 int sq_synchronize(sq_ctx *ctx);
 
@@ -184,6 +198,9 @@ typedef struct {
     double tol_poisson;      /* <= 0: TOL_POISSON */
     const double *rho0;      /* initial density on the cell; NULL: superposed atomic densities */
     int verbose;             /* 1: one line per iteration on stdout */
+    double exit_tol;         /* > 0: per-node early exit (sq_set_early_exit) from the second
+                                iteration on, at the previous iteration's Fermi level */
+    int exit_min, exit_every;   /* its first check and spacing (<= 0: 30 and 5) */
 } sq_scf_options;
 
 typedef struct {
@@ -193,6 +210,7 @@ typedef struct {
     /* e_total = e_band + e_xc - e_vxc + e_el - e_self + minus_ts (Ha), with
        e_el = 1/2 int phi (b - rho); energies of the last input density */
     double wall_s, sq_s, poisson_s;   /* SCF wall time; SQ passes + Fermi level; Poisson solves */
+    double mean_rule_size;   /* mean Gauss nodes per node over the SQ passes */
 } sq_scf_result;
 
 enum sq_field {

@@ -26,7 +26,7 @@ EXPORTS = ["sq_last_error", "sq_default_options", "sq_create_from_deck", "sq_des
            "sq_run", "sq_get_nodes_and_weights", "sq_get_lanczos", "sq_nodes_and_weights", "sq_last_timing",
            "sq_apply_h", "sq_apply_nonlocal", "sq_tridiag_nodes_weights", "sq_density_fermi_energy", "sq_synchronize",
            "sq_get_image", "sq_default_scf_options", "sq_scf", "sq_get_cell_field", "sq_set_veff_cell", "sq_xc",
-           "sq_poisson"]
+           "sq_poisson", "sq_set_early_exit", "sq_get_rule_sizes"]
 
 FIELDS = {"rho": 0, "rho_out": 1, "phi": 2, "vxc": 3, "veff": 4, "b": 5, "rho_atomic": 6}
 
@@ -45,7 +45,8 @@ class Options(C.Structure):
 class ScfOptions(C.Structure):
     _fields_ = [("nonlocal_mode", C.c_int), ("precision", C.c_int), ("max_iter", C.c_int), ("tol", C.c_double),
                 ("mixing", C.c_double), ("history", C.c_int), ("pulay_frequency", C.c_int),
-                ("tol_poisson", C.c_double), ("rho0", C.POINTER(C.c_double)), ("verbose", C.c_int)]
+                ("tol_poisson", C.c_double), ("rho0", C.POINTER(C.c_double)), ("verbose", C.c_int),
+                ("exit_tol", C.c_double), ("exit_min", C.c_int), ("exit_every", C.c_int)]
 
 
 class ScfResult(C.Structure):
@@ -53,7 +54,7 @@ class ScfResult(C.Structure):
                 ("residual", C.c_double), ("lambda_f", C.c_double), ("e_band", C.c_double),
                 ("minus_ts", C.c_double), ("e_xc", C.c_double), ("e_vxc", C.c_double), ("e_el", C.c_double),
                 ("e_self", C.c_double), ("e_total", C.c_double), ("wall_s", C.c_double), ("sq_s", C.c_double),
-                ("poisson_s", C.c_double)]
+                ("poisson_s", C.c_double), ("mean_rule_size", C.c_double)]
 
 
 def deck_grid(directory, name):
@@ -107,6 +108,8 @@ def lib():
     L.sq_set_veff_cell.argtypes = [vp, dp]
     L.sq_xc.argtypes = [vp, dp, lg, dp, dp]
     L.sq_poisson.argtypes = [vp, dp, C.c_double, dp, C.POINTER(C.c_int)]
+    L.sq_set_early_exit.argtypes = [vp, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int]
+    L.sq_get_rule_sizes.argtypes = [vp, ip]
     L.sq_tridiag_nodes_weights.argtypes = [vp, dp, dp, C.c_int, lg, dp, dp, dp, dp]
     L.sq_density_fermi_energy.argtypes = [vp, C.c_double, C.c_double, dp, dp, dp, dp]
     L.sq_synchronize.argtypes = [vp]
@@ -248,10 +251,21 @@ class Context:
         I = self.info
         return (I.npts[2] - 1, I.npts[1] - 1, I.npts[0] - 1)
 
+    def set_early_exit(self, tol, lambda_f, beta=0.0, min_steps=30, every=5):
+        """Per-node early exit of the next sq_run calls (tol <= 0: off)."""
+        _check(lib().sq_set_early_exit(self._h, float(tol), float(lambda_f), float(beta), int(min_steps), int(every)))
+
+    def rule_sizes(self):
+        """Gauss nodes per node of the last run."""
+        n = np.zeros(self._last[1], np.int32)
+        _check(lib().sq_get_rule_sizes(self._h, n.ctypes.data_as(C.POINTER(C.c_int))))
+        return n
+
     def scf(self, mode=SQ_NL_KB, precision=SQ_FP64, max_iter=0, tol=0.0, mixing=0.0, history=0,
-            pulay_frequency=0, tol_poisson=0.0, rho0=None, verbose=False):
+            pulay_frequency=0, tol_poisson=0.0, rho0=None, verbose=False, exit_tol=0.0, exit_min=0, exit_every=0):
         o = ScfOptions()
         lib().sq_default_scf_options(C.byref(o))
+        o.exit_tol, o.exit_min, o.exit_every = float(exit_tol), int(exit_min), int(exit_every)
         o.nonlocal_mode, o.precision, o.max_iter = int(mode), int(precision), int(max_iter)
         o.tol, o.mixing, o.history, o.pulay_frequency = float(tol), float(mixing), int(history), int(pulay_frequency)
         o.tol_poisson, o.verbose = float(tol_poisson), int(bool(verbose))

@@ -75,7 +75,12 @@ struct sq_ctx {
     void *scratch = nullptr;
     double *a = nullptr, *b = nullptr, *lam = nullptr, *w = nullptr, *lmin = nullptr, *lmax = nullptr, *qlws = nullptr;
     int *status = nullptr;
+    int *queue = nullptr, *nsteps = nullptr;   // z-march work queue (8 counters); Gauss nodes per node
     double *partial = nullptr, *partial2 = nullptr, *out = nullptr;
+    // per-node early exit of the z-march kernel (exit_tol <= 0: off)
+    double exit_tol = 0, exit_lf = 0, exit_beta = 0;
+    int exit_min = 30, exit_every = 5;
+    bool last_exit = false;   // the last sq_run used it
     // last run
     long last_begin = 0, last_count = 0;
     bool have_run = false;
@@ -108,7 +113,8 @@ void free_ctx(sq_ctx *c)
     if (!c) return;
     void *ptrs[] = {c->img_dimp, c->rad64, c->rad32, c->img_rad, c->img_cen, c->chi64, c->chi32, c->img_lo, c->img_dim, c->img_off, c->img_atom, c->img_type, c->gamma,
                     c->cov_off, c->cov_img, c->diag64, c->diag32, c->c64, c->c32, c->scratch, c->a, c->b,
-                    c->lam, c->w, c->lmin, c->lmax, c->qlws, c->status, c->partial, c->partial2, c->out};
+                    c->lam, c->w, c->lmin, c->lmax, c->qlws, c->status, c->partial, c->partial2, c->out,
+                    c->queue, c->nsteps};
     void *scf_ptrs[] = {c->scf.rho, c->scf.rho_out, c->scf.b, c->scf.phi, c->scf.eps, c->scf.vxc, c->scf.veff, c->scf.r,
                         c->scf.p, c->scf.Ap, c->scf.f, c->scf.x_prev, c->scf.f_prev, c->scf.dX, c->scf.dF, c->scf.g,
                         c->scf.s, c->scf.part};
@@ -282,6 +288,8 @@ int create_device(sq_ctx *c, int device)
     HIP_TRY(hipMalloc((void **)&c->partial, kRedBlocks * sizeof(double)));
     HIP_TRY(hipMalloc((void **)&c->partial2, kRedBlocks * sizeof(double)));
     HIP_TRY(hipMalloc((void **)&c->out, 4 * sizeof(double)));
+    HIP_TRY(hipMalloc((void **)&c->queue, 8 * sizeof(int)));
+    HIP_TRY(hipMalloc((void **)&c->nsteps, std::max<int64_t>(1, nn) * sizeof(int)));
     // LDS layout of the Lanczos workgroup
     c->img_cap = std::max(1, max_images_per_cube(S));
     c->nlm_tmpl = S.max_nlm <= 6 ? 6 : kMaxChannels;
@@ -415,8 +423,16 @@ KParams make_params(const sq_ctx *c, int mode, int precision, long begin, long c
         P.lds_tcen = c->z_tcen;
         P.lds_plane = c->z_plane;
         P.kmax = c->kmax;
-        P.xcd_map = 1;
         P.zwg = c->zwg[precision];
+        P.queue = c->queue;
+        P.nsteps = c->nsteps;
+        if (c->exit_tol > 0) {
+            P.exit_tol = c->exit_tol;
+            P.exit_lf = c->exit_lf;
+            P.exit_beta = c->exit_beta > 0 ? c->exit_beta : S.beta;
+            P.exit_min = std::max(2, c->exit_min);
+            P.exit_every = std::max(1, c->exit_every);
+        }
     }
     P.scratch = c->scratch;
     P.out_a = c->a;
@@ -463,8 +479,10 @@ int finish_device_setup(sq_ctx *c)
                 KParams P = make_params(c, SQ_NL_KB, prec, 0, std::min<long>(grids[prec][sh], S.n_nodes()), true);
                 P.zwg = sh ? 1024 : 512;
                 P.nqp = std::min(P.nqp, 5);
+                P.exit_tol = 0;
                 const int grid = (int)std::min<long>(grids[prec][sh], std::max<long>(8, (P.node_count + 7) & ~7L));
                 for (int rep = 0; rep < 2; rep++) {   // the first launch loads the code object
+                    HIP_TRY(hipMemsetAsync(c->queue, 0, 8 * sizeof(int), c->stream));
                     HIP_TRY(hipEventRecord(c->ev[0], c->stream));
                     HIP_TRY(launch_zmarch(P, prec, grid, c->zlds, c->stream));
                     HIP_TRY(hipEventRecord(c->ev[1], c->stream));
@@ -612,10 +630,13 @@ int sq_run(sq_ctx *c, int mode, int precision, long begin, long count)
     HIP_TRY(hipSetDevice(c->device));
     c->have_run = false;
     HIP_TRY(hipEventRecord(c->ev[0], c->stream));
+    c->last_exit = false;
     if (c->zmarch) {
         KParams P = make_params(c, mode, precision, begin, count, true);
         int grid = c->zgrid[precision];
         if (count < grid) grid = std::max<long>(8, (count + 7) & ~7L);
+        c->last_exit = P.exit_tol > 0;
+        HIP_TRY(hipMemsetAsync(c->queue, 0, 8 * sizeof(int), c->stream));
         HIP_TRY(launch_zmarch(P, precision, grid, c->zlds, c->stream));
     } else {
         KParams P = make_params(c, mode, precision, begin, count);
@@ -629,7 +650,7 @@ int sq_run(sq_ctx *c, int mode, int precision, long begin, long count)
     HIP_TRY(hipEventRecord(c->ev[2], c->stream));
     HIP_TRY(launch_ql(c->a + begin * (S.nqp + 1), c->b + begin * (S.nqp + 1), S.nqp, S.nqp + 1, count,
                       c->lam + begin * S.nqp, c->w + begin * S.nqp, c->lmin + begin, c->lmax + begin, c->status + begin,
-                      c->qlws, c->stream));
+                      c->qlws, c->stream, c->last_exit ? c->nsteps + begin : nullptr));
     HIP_TRY(hipEventRecord(c->ev[3], c->stream));
     HIP_TRY(hipEventSynchronize(c->ev[3]));
     if ((rc = check_status(c, begin, count, "QL"))) return rc;
@@ -849,6 +870,9 @@ int sq_density_fermi_energy(sq_ctx *c, double nelec, double beta, double *lambda
 // until ||rho_out - rho_in|| / ||rho_out|| < TOL_SCF, with the Harris-Foulkes
 // energy E = E_band + E_xc - int V_xc rho + 1/2 int phi (b - rho) - E_self - TS.
 
+int sq_get_rule_sizes(sq_ctx *c, int *n);
+int sq_set_early_exit(sq_ctx *c, double tol, double lambda_f, double beta, int min_steps, int every);
+
 static int cell_dims(const sq_ctx *c, int G[3])
 {
     const System &S = c->sys;
@@ -1022,6 +1046,14 @@ int sq_scf(sq_ctx *c, const sq_scf_options *opt_in, sq_scf_result *res)
     const int pf = opt.pulay_frequency > 0 ? opt.pulay_frequency : std::max(1, S.pulay_frequency);
     const double tolp = opt.tol_poisson > 0 ? opt.tol_poisson : S.tol_poisson;
     HIP_TRY(hipSetDevice(c->device));
+    // the loop sets the early exit per pass; the caller's setting comes back after it
+    struct ExitGuard {
+        sq_ctx *c;
+        double tol, lf, beta;
+        int mn, ev;
+        ~ExitGuard() { c->exit_tol = tol; c->exit_lf = lf; c->exit_beta = beta; c->exit_min = mn; c->exit_every = ev; }
+    } guard{c, c->exit_tol, c->exit_lf, c->exit_beta, c->exit_min, c->exit_every};
+    c->exit_tol = 0.0;
     const auto t0 = std::chrono::steady_clock::now();
     if ((rc = scf_alloc(c, m)) || (rc = scf_pseudocharge(c))) return rc;
     auto &F = c->scf;
@@ -1037,7 +1069,7 @@ int sq_scf(sq_ctx *c, const sq_scf_options *opt_in, sq_scf_result *res)
     }
     HIP_TRY(hipMemset(F.phi, 0, n * sizeof(double)));
     memset(res, 0, sizeof *res);
-    double t_sq = 0.0, t_poisson = 0.0;
+    double t_sq = 0.0, t_poisson = 0.0, rule_sum = 0.0;
     int nhist = 0, head = 0;
     const int ext0[3] = {S.ext0[0], S.ext0[1], S.ext0[2]}, extn[3] = {S.extn[0], S.extn[1], S.extn[2]};
     for (int it = 1; it <= maxit; it++) {
@@ -1050,9 +1082,22 @@ int sq_scf(sq_ctx *c, const sq_scf_options *opt_in, sq_scf_result *res)
         t_poisson += std::chrono::duration<double>(std::chrono::steady_clock::now() - tp).count();
         HIP_TRY(launch_axpbypgz(1.0, F.phi, 0.0, F.veff, 1.0, F.vxc, n, c->stream));  // phi + V_xc
         HIP_TRY(launch_veff_to_diag(F.veff, S.cdiag, ext0, extn, G, c->diag64, c->diag32, c->stream));
-        // SQ pass and density
+        // SQ pass (from the second iteration with the per-node early exit at the
+        // previous Fermi level, when asked for) and density
         const auto ts = std::chrono::steady_clock::now();
-        if ((rc = sq_run(c, opt.nonlocal_mode, opt.precision, 0, n))) return rc;
+        if (opt.exit_tol > 0 && it > 1)
+            sq_set_early_exit(c, opt.exit_tol, res->lambda_f, 0.0, opt.exit_min > 0 ? opt.exit_min : 30,
+                              opt.exit_every > 0 ? opt.exit_every : 5);
+        rc = sq_run(c, opt.nonlocal_mode, opt.precision, 0, n);
+        c->exit_tol = 0.0;
+        if (rc) return rc;
+        {
+            std::vector<int> ns(n);
+            if ((rc = sq_get_rule_sizes(c, ns.data()))) return rc;
+            long tot = 0;
+            for (int v : ns) tot += v;
+            rule_sum += (double)tot / n;
+        }
         double lf, eb, mts;
         if ((rc = density_device(c, 0.0, 0.0, &lf, F.rho_out, &eb, &mts))) return rc;
         t_sq += std::chrono::duration<double>(std::chrono::steady_clock::now() - ts).count();
@@ -1109,6 +1154,7 @@ int sq_scf(sq_ctx *c, const sq_scf_options *opt_in, sq_scf_result *res)
     }
     HIP_TRY(hipStreamSynchronize(c->stream));
     res->wall_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    res->mean_rule_size = res->iterations > 0 ? rule_sum / res->iterations : 0.0;
     res->sq_s = t_sq;
     res->poisson_s = t_poisson;
     return SQ_OK;
@@ -1200,6 +1246,30 @@ int sq_poisson(sq_ctx *c, const double *f, double tol, double *phi, int *iters)
     HIP_TRY(hipMemset(F.phi, 0, F.ng * sizeof(double)));
     if ((rc = scf_poisson(c, F.eps, tol > 0 ? tol : c->sys.tol_poisson, 20000, iters))) return rc;
     HIP_TRY(hipMemcpy(phi, F.phi, F.ng * sizeof(double), hipMemcpyDeviceToHost));
+    return SQ_OK;
+}
+
+int sq_set_early_exit(sq_ctx *c, double tol, double lambda_f, double beta, int min_steps, int every)
+{
+    if (!c) return set_err(SQ_ERR_ARG, "null context");
+    if (tol > 0 && (min_steps < 2 || every < 1)) return set_err(SQ_ERR_ARG, "early exit needs min_steps >= 2, every >= 1");
+    c->exit_tol = tol > 0 ? tol : 0.0;
+    c->exit_lf = lambda_f;
+    c->exit_beta = beta;
+    c->exit_min = min_steps;
+    c->exit_every = every;
+    return SQ_OK;
+}
+
+int sq_get_rule_sizes(sq_ctx *c, int *n)
+{
+    if (!c || !n) return set_err(SQ_ERR_ARG, "null argument");
+    if (!c->have_run) return set_err(SQ_ERR_STATE, "no completed sq_run");
+    if (!c->last_exit) {
+        for (long r = 0; r < c->last_count; r++) n[r] = c->sys.nqp;
+        return SQ_OK;
+    }
+    HIP_TRY(hipMemcpy(n, c->nsteps + c->last_begin, c->last_count * sizeof(int), hipMemcpyDeviceToHost));
     return SQ_OK;
 }
 

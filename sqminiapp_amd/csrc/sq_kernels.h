@@ -7,6 +7,7 @@ namespace sqamd {
 
 constexpr int kMaxChannels = 16;  // (l,m) channels per type for lmax <= 3
 constexpr int kMaxNqp = 256;      // Lanczos degree limit of the QL kernel
+constexpr int kMatsubara = 8;     // Matsubara points of the early-exit test
 constexpr int kMaxTypesDev = 8;
 
 // Everything a Lanczos / apply kernel reads; passed by value.
@@ -30,7 +31,6 @@ struct KParams {
     const double4 *img_cen;                       // image position (X0,Y0,Z0,-)
     double hgrid[3], range[3];                    // x = h*gx - R - X0 (lsdft_nl.cc:174-190)
     signed char ch_l[kMaxTypesDev][kMaxChannels], ch_m[kMaxTypesDev][kMaxChannels], ch_r[kMaxTypesDev][kMaxChannels];
-    int xcd_map;                                  // 1: nodes of one pass split contiguously per XCD
     int layout;                                   // 1: every type has nonlocal l = {0, 2} (s+d channels)
     int zwg;                                      // z-march workgroup: 512 (1-2 columns per thread) or 1024 (1)
     // stencil weights cx[0..order] per axis (element type = precision)
@@ -49,6 +49,14 @@ struct KParams {
     void *scratch;               // 3*nc elements per workgroup
     double *out_a, *out_b;       // [node][nqp+1]
     int *status;                 // [node]
+    // work queue of the z-march kernel: one counter per XCD, zeroed before a launch
+    int *queue;
+    // per-node early exit: stop when the lowest kMatsubara Matsubara terms
+    // (2/beta) g(lambda_f + i w_m) of the node's quadrature change by < exit_tol
+    // between checks at n = exit_min, exit_min + exit_every, ... (exit_tol <= 0: off)
+    double exit_tol, exit_lf, exit_beta;
+    int exit_min, exit_every;
+    int *nsteps;                 // [node] Gauss nodes of the node's rule
 };
 
 hipError_t launch_lanczos(const KParams &P, int precision, int grid, size_t lds, hipStream_t st);
@@ -57,7 +65,7 @@ int zmarch_occupancy(const KParams &P, int precision, size_t lds);
 hipError_t launch_apply(const KParams &P, long id, const double *x, double *y, int nonlocal_only, size_t lds,
                         hipStream_t st);
 hipError_t launch_ql(const double *a, const double *b, int n, int stride, long count, double *lam, double *w,
-                     double *lmin, double *lmax, int *status, double *ws, hipStream_t st);
+                     double *lmin, double *lmax, int *status, double *ws, hipStream_t st, const int *nsteps = nullptr);
 hipError_t launch_occupation(const double *lam, const double *w, long nq, double beta, double lf, double *partial,
                              int nblocks, double *out, hipStream_t st);
 hipError_t launch_density_energy(const double *lam, const double *w, long n_nodes, int nqp, double beta, double lf,

@@ -208,14 +208,17 @@ __global__ void __launch_bounds__(kWG) apply_kernel(KParams P, long id, const T 
 // lsdft_tools.cc:912-1026).  Only row 0 of the eigenvector matrix is carried:
 // the reference's rotations update each row of z independently, so row 0 -- the
 // one the weights w_i = z[0][i]^2 use -- is the same number sequence.
-__global__ void ql_kernel(const double *a, const double *bsub, int n, int stride, long count, double *lam, double *w,
-                          double *lmin, double *lmax, int *status, double *ws)
+__global__ void ql_kernel(const double *a, const double *bsub, int nmax, int stride, long count, double *lam, double *w,
+                          double *lmin, double *lmax, int *status, double *ws, const int *nsteps)
 {
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= count) return;
+    // a node that stopped early has a Gauss rule of nsteps[t] nodes; the rest of
+    // its row is padded with weight zero at its largest node
+    const int n = nsteps ? max(1, min(nmax, nsteps[t])) : nmax;
     // d, e and row 0 of z live in a workspace laid out [i][thread] so that the
     // threads of a wavefront touch consecutive addresses.
-    double *d = ws + t, *e = ws + (long)n * count + t, *z = ws + 2L * n * count + t;
+    double *d = ws + t, *e = ws + (long)nmax * count + t, *z = ws + 2L * nmax * count + t;
 #define D(i) d[(long)(i) * count]
 #define EE(i) e[(long)(i) * count]
 #define Z(i) z[(long)(i) * count]
@@ -270,10 +273,14 @@ __global__ void ql_kernel(const double *a, const double *bsub, int n, int stride
     double emin = D(0), emax = D(0);
     for (i = 0; i < n; i++) {
         const double di = D(i), zi = Z(i);
-        lam[t * n + i] = di;
-        w[t * n + i] = zi * zi;
+        lam[t * nmax + i] = di;
+        w[t * nmax + i] = zi * zi;
         if (di > emax) emax = di;
         else if (di < emin) emin = di;
+    }
+    for (i = n; i < nmax; i++) {
+        lam[t * nmax + i] = emax;
+        w[t * nmax + i] = 0.0;
     }
 #undef D
 #undef EE
@@ -409,13 +416,13 @@ hipError_t launch_apply(const KParams &P, long id, const double *x, double *y, i
 }
 
 hipError_t launch_ql(const double *a, const double *b, int n, int stride, long count, double *lam, double *w,
-                     double *lmin, double *lmax, int *status, double *ws, hipStream_t st)
+                     double *lmin, double *lmax, int *status, double *ws, hipStream_t st, const int *nsteps)
 {
     if (n < 1 || n > kMaxNqp || count <= 0) return hipErrorInvalidValue;
     const int threads = 64;
     const long blocks = (count + threads - 1) / threads;
     hipLaunchKernelGGL(ql_kernel, dim3((unsigned)blocks), dim3(threads), 0, st, a, b, n, stride, count, lam, w, lmin,
-                       lmax, status, ws);
+                       lmax, status, ws, nsteps);
     return hipGetLastError();
 }
 

@@ -114,12 +114,34 @@ __global__ void __launch_bounds__(WG) lanczos_zmarch(KParams P)
     // Nodes are taken in passes of gridDim.x; within a pass the workgroups of
     // one XCD (blockIdx % 8 under round-robin dispatch) take a contiguous run,
     // so the projector tables they share stay in that XCD's L2.
-    const int G = gridDim.x;
-    const int xcd = blockIdx.x & 7, slot = blockIdx.x >> 3;
-    const bool xmap = P.xcd_map && (G & 7) == 0;
-    for (long pass = 0; pass * G < P.node_count; pass++) {
-        const long r = pass * G + (xmap ? (long)xcd * (G >> 3) + slot : (long)blockIdx.x);
-        if (r >= P.node_count) continue;
+    // Work queue: the node range is cut into eight contiguous parts, one per
+    // XCD (blockIdx % 8 under round-robin dispatch); a workgroup takes the next
+    // node of its XCD's part with an atomic counter, so the projector tables its
+    // neighbours share stay in that XCD's L2, and moves on to the other parts
+    // when its own is done -- nodes that stop early leave no workgroup idle.
+    const long part = (P.node_count + 7) / 8;
+    const int home = blockIdx.x & 7;
+    int q_at = 0;                       // parts tried so far (thread 0)
+    __shared__ long s_next;
+    __shared__ double s_ab[2][kMaxNqp + 1];
+    __shared__ double s_g[kMatsubara][2];
+    __shared__ int s_flag;
+    for (;;) {
+        if (tid == 0) {
+            long r = -1;
+            for (; q_at < 8; q_at++) {
+                const int x = (home + q_at) & 7;
+                const long lo = x * part, hi = min(P.node_count, lo + part);
+                if (lo >= hi) continue;
+                const long k = atomicAdd(&P.queue[x], 1);
+                if (lo + k < hi) { r = lo + k; break; }
+            }
+            s_next = r;
+        }
+        __syncthreads();
+        const long r = s_next;
+        __syncthreads();
+        if (r < 0) break;
         const long id = P.node_begin + r;
         int node[3];
         node_coords(P, id, node);
@@ -203,6 +225,7 @@ __global__ void __launch_bounds__(WG) lanczos_zmarch(KParams P)
         double bprev = 0.0;
         double *out_a = P.out_a + id * (P.nqp + 1), *out_b = P.out_b + id * (P.nqp + 1);
         int status = 0;
+        int nq = P.nqp, checks = 0;   // Gauss nodes of the node's rule; early-exit checks made
         for (int step = 0; step <= P.nqp; step++) {
             // ---- gather: A[li][ch] = sum_{box & cube} v chi
             for (int li = wave; li < nimg; li += NW) {
@@ -503,12 +526,50 @@ __global__ void __launch_bounds__(WG) lanczos_zmarch(KParams P)
             const double b = sqrt(bb);
             if (tid == 0) { out_a[step] = a; out_b[step] = b; }
             if (b == 0.0) { status = 7; break; }
+            if (P.exit_tol > 0.0) {
+                // Early exit: the node's Gauss rule of n nodes is J_n (a_0..a_{n-1},
+                // b_0..b_{n-2}); its resolvent g(z) = e1^T (z - J_n)^{-1} e1 is the
+                // continued fraction 1/(z - a_0 - b_0^2/(z - a_1 - ...)).  The
+                // Fermi quadrature is 1/2 + (2/beta) sum_m Re g(lambda_f + i w_m)
+                // with w_m = (2m+1) pi / beta; the lowest terms converge last.
+                if (tid == 0) { s_ab[0][step] = a; s_ab[1][step] = b; }
+                const int n = step + 1;
+                if (n >= P.exit_min && (n - P.exit_min) % P.exit_every == 0 && n < P.nqp) {
+                    if (tid == 0) s_flag = 0;
+                    __syncthreads();
+                    if (tid < kMatsubara) {
+                        const double wm = (2 * tid + 1) * M_PI / P.exit_beta;
+                        double tr = P.exit_lf - s_ab[0][n - 1], ti = wm;
+                        for (int j = n - 2; j >= 0; j--) {
+                            const double den = tr * tr + ti * ti;   // >= wm^2 > 0
+                            const double bb = s_ab[1][j] * s_ab[1][j];
+                            tr = (P.exit_lf - s_ab[0][j]) - bb * tr / den;
+                            ti = wm + bb * ti / den;
+                        }
+                        const double den = tr * tr + ti * ti;
+                        const double gr = tr / den, gi = -ti / den;
+                        const double dr = gr - s_g[tid][0], di = gi - s_g[tid][1];
+                        const double d = (2.0 / P.exit_beta) * sqrt(dr * dr + di * di);
+                        if (checks == 0 || !(d < P.exit_tol)) s_flag = 1;
+                        s_g[tid][0] = gr;
+                        s_g[tid][1] = gi;
+                    }
+                    __syncthreads();
+                    const bool stop = s_flag == 0;
+                    checks++;
+                    __syncthreads();
+                    if (stop) { nq = n; break; }
+                }
+            }
             T *t = Bv; Bv = Bvp; Bvp = t;
             svp = sv;
             sv = T(1.0 / b);
             bprev = b;
         }
-        if (tid == 0) P.status[id] = status;
+        if (tid == 0) {
+            P.status[id] = status;
+            if (P.nsteps) P.nsteps[id] = nq;
+        }
         __syncthreads();
     }
 }

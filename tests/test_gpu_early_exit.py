@@ -1,0 +1,65 @@
+"""Per-node early exit of the column-marching Lanczos kernel (sq_set_early_exit).
+
+A node that stops after n steps has the Gauss rule of n nodes: its nodes and
+weights equal, bit for bit, those of a run with LANCZOS_DEGREE = n (the
+recurrence is the same up to step n-1 and QL sees the same tridiagonal), and
+the rest of its row carries weight zero.  With the exit off, results are those
+of a run that never set it.  (The stopping test itself is pinned on the CPU
+against closed forms, tests/test_matsubara.py.)
+"""
+import numpy as np
+import pytest
+
+from conftest import MG1, write_variant
+
+import sqminiapp_amd as sq
+
+pytestmark = pytest.mark.gpu
+
+
+def test_stopped_nodes_equal_shorter_fixed_runs(tmp_path):
+    d, n = write_variant(str(tmp_path), "L", nqp=40)
+    with sq.Context(d, n) as c:
+        c.run(sq.SQ_NL_KB, sq.SQ_FP64)
+        lf, _, _, _ = c.density_fermi_energy()
+        c.set_early_exit(1e-3, lf, 0.0, 12, 4)
+        c.run(sq.SQ_NL_KB, sq.SQ_FP64)
+        lam, w, lmin, lmax = c.results()
+        sizes = c.rule_sizes()
+    print("rule sizes:", np.bincount(sizes)[12:].tolist())
+    assert sizes.min() >= 12 and sizes.max() <= 40 and np.all(((sizes - 12) % 4 == 0) | (sizes == 40))
+    assert np.any(sizes < 40)
+    checked = 0
+    for s in sorted(set(sizes.tolist()))[:3]:
+        ids = np.nonzero(sizes == s)[0][:4]
+        with sq.Context(d, n, nqp=int(s)) as cs:
+            for i in ids:
+                cs.run(sq.SQ_NL_KB, sq.SQ_FP64, begin=int(i), count=1)
+                ls, ws, mn, mx = cs.results()
+                assert np.array_equal(lam[i, :s], ls[0]) and np.array_equal(w[i, :s], ws[0]), (i, s)
+                assert lmin[i] == mn[0] and lmax[i] == mx[0]
+                assert np.all(w[i, s:] == 0.0)
+                checked += 1
+    assert checked > 0
+
+
+def test_exit_off_leaves_results_unchanged():
+    with sq.Context(MG1, "Mg1", phantom_atom=True) as c:
+        c.run(sq.SQ_NL_KB, sq.SQ_FP64)
+        l1, w1, _, _ = c.results()
+        c.set_early_exit(1e-3, 0.5, 0.0, 4, 2)
+        c.set_early_exit(0.0, 0.0)
+        c.run(sq.SQ_NL_KB, sq.SQ_FP64)
+        l2, w2, _, _ = c.results()
+        assert np.all(c.rule_sizes() == c.info.nqp)
+    assert np.array_equal(l1, l2) and np.array_equal(w1, w2)
+
+
+def test_scf_with_early_exit_converges():
+    with sq.Context(MG1, "Mg1", cell=True) as c:
+        r0 = c.scf(sq.SQ_NL_KB, sq.SQ_FP64, tol=1e-6, max_iter=100, mixing=0.3)
+        r = c.scf(sq.SQ_NL_KB, sq.SQ_FP64, tol=1e-6, max_iter=100, mixing=0.3, exit_tol=1e-4, exit_min=4, exit_every=2)
+    de = abs(r["e_total"] - r0["e_total"]) / 4
+    print("Mg1 SCF with early exit:", r["iterations"], "iterations, mean rule", r["mean_rule_size"], "dE/atom", de)
+    assert r["converged"] and r["mean_rule_size"] <= 10 and r0["mean_rule_size"] == 10
+    assert de < 1e-3
