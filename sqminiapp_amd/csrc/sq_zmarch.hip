@@ -66,6 +66,7 @@ __global__ void __launch_bounds__(WG) lanczos_zmarch(KParams P)
     constexpr int U = WL ? (sizeof(T) == 8 ? 4 : 8) : (sizeof(T) == 8 ? 8 : 16);
     constexpr int UG = U;
     constexpr int U0 = 4;   // the generic channel layout holds up to four radial factors per point
+    constexpr int UC = 8;   // planes of a column chunk in the scatter and update
     extern __shared__ __align__(16) unsigned char smem[];
     int *img_list = reinterpret_cast<int *>(smem);
     int4 *tlo = reinterpret_cast<int4 *>(smem + P.lds_tlo);
@@ -421,9 +422,10 @@ __global__ void __launch_bounds__(WG) lanczos_zmarch(KParams P)
             double a;
             if constexpr (sizeof(T) == 8) a = block_sum<NW>(pa + pn, red);
             else a = block_csum<NW>(ca, red) + block_sum<NW>(pn, red);
-            // ---- per column: nonlocal chi*alpha of the column's images, ascending,
-            // added onto the local part, then w = (u - a v) - bprev vprev over
-            // vprev's storage and ||w||^2
+            // ---- per column, in chunks of UC planes held in registers: the local
+            // part of u from the march, plus chi*alpha of every image of the column
+            // that meets the chunk (ascending images), then w = (u - a v) - bprev
+            // vprev over vprev's storage and ||w||^2.  u is never written back.
             const T ta = T(-a), tb = T(-bprev);
             double pb = 0.0;
             CSum cb{0.f, 0.f};
@@ -431,93 +433,101 @@ __global__ void __launch_bounds__(WG) lanczos_zmarch(KParams P)
             for (int k = 0; k < CPT; k++) {
                 if (!has[k]) continue;
                 const int c = col[k];
+                const T x = (T)(P.hgrid[0] * gxc[k] - P.range[0]);   // minus the image centre below
+                const T y = (T)(P.hgrid[1] * gyc[k] - P.range[1]);
 #pragma unroll 1
-                for (int w = 0; w < MW; w++) {
-                    // mask word w by a select chain: no dynamic register indexing
-                    unsigned m = cmask[k][0];
+                for (int z0 = 0; z0 < nz; z0 += UC) {
+                    T uu[UC];
 #pragma unroll
-                    for (int q = 1; q < MW; q++) m = (w == q) ? cmask[k][q] : m;
-                    while (m) {
-                        const int li = (w << 5) + __ffs(m) - 1;
-                        m &= m - 1u;
-                        const int4 lo = tlo[li], dm = tdim[li];
-                        const int nlm = dm.w & 255, nl = (dm.w >> 8) & 255, type = dm.w >> 16;
-                        const double4 cen = tcen[li];
-                        const int zlo = max(lo.z, cube_lo[2]) - cube_lo[2];
-                        const int zhi = min(lo.z + dm.z - 1, cube_lo[2] + nz - 1) - cube_lo[2];
-                        const int64_t rb = toff[li] +
-                                           ((int64_t)((cube_lo[2] + zlo - lo.z) * dm.y + (gyc[k] - lo.y)) * dm.x + (gxc[k] - lo.x)) * nl;
-                        const T x = (T)(P.hgrid[0] * gxc[k] - P.range[0] - cen.x);
-                        const T y = (T)(P.hgrid[1] * gyc[k] - P.range[1] - cen.y);
-                        T al[NLM];
-#pragma unroll
-                        for (int ch = 0; ch < NLM; ch++) al[ch] = ch < nlm ? (T)alpha[li * NLM + ch] : T(0);
-                        T *bu = Bu + c + zlo * ncol;
-                        const int nzr = zhi - zlo + 1;
-                        if constexpr (LAYOUT == 1) {
-                            // chi . alpha along the column = t0 al0 + t2 (p0 + p1 z + p2 z^2)
-                            using V2 = typename std::conditional<sizeof(T) == 8, double2, float2>::type;
-                            const SdColumn<T> pz = sd_column(x, y);
-                            const T C20 = T(0.315391565252520);
-                            const T p0 = fma(pz.cxy, al[1], fma(pz.cxxyy, al[5], C20 * (pz.q * al[3])));
-                            const T p1 = fma(pz.cy, al[2], pz.cx * al[4]);
-                            const T p2 = T(2) * C20 * al[3];
-                            const V2 *r2 = reinterpret_cast<const V2 *>(rad + rb);
-                            const int zs2 = dm.x * dm.y;
+                    for (int u = 0; u < UC; u++) uu[u] = (z0 + u < nz) ? Bu[c + (z0 + u) * ncol] : T(0);
 #pragma unroll 1
-                            for (int i = 0; i < nzr; i += U) {
-                                T s4[U];
-                                V2 t4[U];
+                    for (int wd = 0; wd < MW; wd++) {
+                        // mask word wd by a select chain: no dynamic register indexing
+                        unsigned m = cmask[k][0];
 #pragma unroll
-                                for (int u = 0; u < U; u++) {
-                                    const bool ok = i + u < nzr;
-                                    s4[u] = ok ? bu[(i + u) * ncol] : T(0);
-                                    t4[u] = ok ? r2[(i + u) * zs2] : V2{T(0), T(0)};
+                        for (int q = 1; q < MW; q++) m = (wd == q) ? cmask[k][q] : m;
+                        while (m) {
+                            const int li = (wd << 5) + __ffs(m) - 1;
+                            m &= m - 1u;
+                            const int4 lo = tlo[li], dm = tdim[li];
+                            const int zlo = max(lo.z, cube_lo[2]) - cube_lo[2];
+                            const int zhi = min(lo.z + dm.z - 1, cube_lo[2] + nz - 1) - cube_lo[2];
+                            if (zhi < z0 || zlo > z0 + UC - 1) continue;
+                            const int nlm = dm.w & 255, nl = (dm.w >> 8) & 255, type = dm.w >> 16;
+                            const double4 cen = tcen[li];
+                            // radial factors of this column at plane z0 (entries outside the
+                            // image's z range are never loaded)
+                            const int64_t rb0 = toff[li] +
+                                                ((int64_t)((cube_lo[2] + z0 - lo.z) * dm.y + (gyc[k] - lo.y)) * dm.x +
+                                                 (gxc[k] - lo.x)) * nl;
+                            const T xc = x - (T)cen.x, yc = y - (T)cen.y;
+                            T al[NLM];
+#pragma unroll
+                            for (int ch = 0; ch < NLM; ch++) al[ch] = ch < nlm ? (T)alpha[li * NLM + ch] : T(0);
+                            const T zb = (T)(P.hgrid[2] * (cube_lo[2] + z0) - P.range[2] - cen.z);
+                            if constexpr (LAYOUT == 1) {
+                                // chi . alpha along the column = t0 al0 + t2 (p0 + p1 z + p2 z^2)
+                                using V2 = typename std::conditional<sizeof(T) == 8, double2, float2>::type;
+                                const SdColumn<T> pz = sd_column(xc, yc);
+                                const T C20 = T(0.315391565252520);
+                                const T p0 = fma(pz.cxy, al[1], fma(pz.cxxyy, al[5], C20 * (pz.q * al[3])));
+                                const T p1 = fma(pz.cy, al[2], pz.cx * al[4]);
+                                const T p2 = T(2) * C20 * al[3];
+                                const V2 *r2 = reinterpret_cast<const V2 *>(rad + rb0);
+                                const int zs2 = dm.x * dm.y;
+                                V2 t4[UC];
+#pragma unroll
+                                for (int u = 0; u < UC; u++) {
+                                    const bool ok = z0 + u >= zlo && z0 + u <= zhi;
+                                    t4[u] = ok ? r2[u * zs2] : V2{T(0), T(0)};
                                 }
-                                const T zb = (T)(P.hgrid[2] * (cube_lo[2] + zlo + i) - P.range[2] - cen.z);
 #pragma unroll
-                                for (int u = 0; u < U; u++) {
+                                for (int u = 0; u < UC; u++) {
                                     const T z = fma(hzt, T(u), zb);
                                     const T h = fma(z, fma(z, p2, p1), p0);
-                                    if (i + u < nzr) bu[(i + u) * ncol] = fma(t4[u].y, h, fma(t4[u].x, al[0], s4[u]));
+                                    if (z0 + u >= zlo && z0 + u <= zhi) uu[u] = fma(t4[u].y, h, fma(t4[u].x, al[0], uu[u]));
                                 }
-                            }
-                        } else {
-                            const T *r0 = rad + rb;
-                            const int64_t zs = (int64_t)dm.x * dm.y * nl;
+                            } else {
+                                const int64_t zs = (int64_t)dm.x * dm.y * nl;
 #pragma unroll 1
-                            for (int i = 0; i < nzr; i++) {
-                                T tt[4];
+                                for (int u = 0; u < UC; u++) {
+                                    if (z0 + u < zlo || z0 + u > zhi) continue;
+                                    T tt[4];
 #pragma unroll
-                                for (int q = 0; q < 4; q++) tt[q] = q < nl ? r0[i * zs + q] : T(0);
-                                const T zc = (T)(P.hgrid[2] * (cube_lo[2] + zlo + i) - P.range[2] - cen.z);
-                                T cv[NLM];
-                                chi_channels<T, NLM, LAYOUT>(P, type, nlm, tt, x, y, zc, cv);
-                                T sum = bu[i * ncol];
+                                    for (int q = 0; q < 4; q++) tt[q] = q < nl ? rad[rb0 + u * zs + q] : T(0);
+                                    const T zc = fma(hzt, T(u), zb);
+                                    T cv[NLM];
+                                    chi_channels<T, NLM, LAYOUT>(P, type, nlm, tt, xc, yc, zc, cv);
+                                    T sum = uu[0];
 #pragma unroll
-                                for (int ch = 0; ch < NLM; ch++)
-                                    if (ch < nlm) sum = fma(cv[ch], al[ch], sum);
-                                bu[i * ncol] = sum;
+                                    for (int q = 0; q < UC; q++) sum = (q == u) ? uu[q] : sum;
+#pragma unroll
+                                    for (int ch = 0; ch < NLM; ch++)
+                                        if (ch < nlm) sum = fma(cv[ch], al[ch], sum);
+#pragma unroll
+                                    for (int q = 0; q < UC; q++) uu[q] = (q == u) ? sum : uu[q];
+                                }
                             }
                         }
                     }
-                }
-                for (int z = 0; z < nz; z++) {
-                    const int p = c + z * ncol;
-                    const T vc = sv * Bv[p];
-                    const T vpc = svp * Bvp[p];
-                    const T u = Bu[p];
-                    T w;
-                    if constexpr (sizeof(T) == 8) {
-                        w = fma(ta, vc, u);
-                        w = fma(tb, vpc, w);
-                        pb = fma(w, w, pb);
-                    } else {
-                        w = fmaf(ta, vc, u);
-                        w = fmaf(tb, vpc, w);
-                        cb = csum_add(cb, w * w);
+#pragma unroll
+                    for (int u = 0; u < UC; u++) {
+                        if (z0 + u >= nz) break;
+                        const int p = c + (z0 + u) * ncol;
+                        const T vc = sv * Bv[p];
+                        const T vpc = svp * Bvp[p];
+                        T w;
+                        if constexpr (sizeof(T) == 8) {
+                            w = fma(ta, vc, uu[u]);
+                            w = fma(tb, vpc, w);
+                            pb = fma(w, w, pb);
+                        } else {
+                            w = fmaf(ta, vc, uu[u]);
+                            w = fmaf(tb, vpc, w);
+                            cb = csum_add(cb, w * w);
+                        }
+                        Bvp[p] = w;
                     }
-                    Bvp[p] = w;
                 }
             }
             double bb;
