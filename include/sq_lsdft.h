@@ -71,8 +71,10 @@ int Initialize(SQ_LSDFT *pLsdft, const char *dir, const char *name, const sq_opt
 int Objects_Destroy(SQ_LSDFT *pLsdft);
 
 /* CalculateNonlocalProjectors (lsdft_nl.cc:141-208) and NonlocalProjectors
- * (lsdft_nl.cc:6-130).  Initialize has already built and uploaded chi and its
- * radial factors; these check that the tables exist and return 0. */
+ * (lsdft_nl.cc:6-130).  Initialize has already built chi and its radial
+ * factors -- on the device, one workgroup per image, when opt->device >= 0;
+ * on the host for a device-less context -- so these check that the tables
+ * exist and return 0. */
 double CalculateNonlocalProjectors(SQ_LSDFT *pLsdft);
 int NonlocalProjectors(SQ_LSDFT *pLsdft);
 

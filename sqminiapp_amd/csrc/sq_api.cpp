@@ -242,12 +242,31 @@ int create_device(sq_ctx *c, int device)
     if ((rc = dev_upload(&c->img_off, off.data(), M))) return rc;
     if ((rc = dev_upload(&c->img_atom, atom.data(), M))) return rc;
     if ((rc = dev_upload(&c->img_type, type.data(), M))) return rc;
-    if ((rc = dev_upload(&c->chi64, S.chi.data(), S.chi.size()))) return rc;
-    std::vector<float> chi32(S.chi.begin(), S.chi.end());
-    if ((rc = dev_upload(&c->chi32, chi32.data(), chi32.size()))) return rc;
-    if ((rc = dev_upload(&c->rad64, S.rad.data(), S.rad.size()))) return rc;
-    std::vector<float> rad32(S.rad.begin(), S.rad.end());
-    if ((rc = dev_upload(&c->rad32, rad32.data(), rad32.size()))) return rc;
+    if (S.chi.empty() && S.chi_total > 0) {
+        // projector tables built on the device (bit-identical to the host fill)
+        HIP_TRY(hipMalloc((void **)&c->chi64, S.chi_total * sizeof(double)));
+        HIP_TRY(hipMalloc((void **)&c->rad64, std::max<int64_t>(1, S.rad_total) * sizeof(double)));
+        HIP_TRY(hipMalloc((void **)&c->chi32, S.chi_total * sizeof(float)));
+        HIP_TRY(hipMalloc((void **)&c->rad32, std::max<int64_t>(1, S.rad_total) * sizeof(float)));
+        int *dst = nullptr, hst = 0;
+        HIP_TRY(hipMalloc((void **)&dst, sizeof(int)));
+        HIP_TRY(hipMemset(dst, 0, sizeof(int)));
+        const int brc = build_projector_tables(S, c->chi64, c->rad64, dst, c->stream);
+        hipError_t e = hipMemcpy(&hst, dst, sizeof(int), hipMemcpyDeviceToHost);
+        (void)hipFree(dst);
+        if (brc || e != hipSuccess) return set_err(SQ_ERR_DEVICE, "device projector tables failed");
+        if (hst) return set_err(SQ_ERR_RANGE, "projector evaluation failed (radius off the table or bad l)");
+        HIP_TRY(launch_to_float(c->chi64, c->chi32, S.chi_total, c->stream));
+        HIP_TRY(launch_to_float(c->rad64, c->rad32, S.rad_total, c->stream));
+        HIP_TRY(hipStreamSynchronize(c->stream));
+    } else {
+        if ((rc = dev_upload(&c->chi64, S.chi.data(), S.chi.size()))) return rc;
+        std::vector<float> chi32(S.chi.begin(), S.chi.end());
+        if ((rc = dev_upload(&c->chi32, chi32.data(), chi32.size()))) return rc;
+        if ((rc = dev_upload(&c->rad64, S.rad.data(), S.rad.size()))) return rc;
+        std::vector<float> rad32(S.rad.begin(), S.rad.end());
+        if ((rc = dev_upload(&c->rad32, rad32.data(), rad32.size()))) return rc;
+    }
     std::vector<int64_t> roff(M);
     std::vector<double4> cen(M);
     for (int g = 0; g < M; g++) {
@@ -536,6 +555,8 @@ int sq_create_from_deck(const char *dir, const char *name, const sq_options *opt
     for (int d = 0; d < 3; d++) so.phantom_pos[d] = opt->phantom_pos[d];
     for (int d = 0; d < 6; d++) so.rank_box[d] = opt->rank_box[d];
     so.nqp_override = opt->nqp_override;
+    // a device context builds chi and rad on the device; SQ_HOST_TABLES=1 builds them on the host
+    so.host_tables = opt->device < 0 || getenv("SQ_HOST_TABLES") != nullptr;
     sq_ctx *c = new sq_ctx();
     Status st = build_system(dir, name, so, c->sys);
     if (!st.ok()) {
@@ -600,7 +621,7 @@ int sq_get_info(sq_ctx *c, sq_info *I)
         for (int l = 0; l < 4; l++) I->gamma[t][l] = S.psd[t].gamma[l];
         for (int d = 0; d < 3; d++) I->cutoff[t][d] = S.cutoff[t][d];
     }
-    I->chi_entries = (long long)S.chi.size();
+    I->chi_entries = (long long)S.chi_total;
     I->kernel = c->zmarch ? 1 : 0;
     I->kmax = c->kmax;
     for (int prec = 0; prec < 2; prec++) {
@@ -1293,7 +1314,14 @@ int sq_get_image(sq_ctx *c, int g, double *center, int *box, int *atom, int *typ
     if (atom) *atom = im.atom;
     if (type) *type = im.type;
     if (entries) *entries = (int)(im.points() * im.nlm);
-    if (chi) memcpy(chi, S.chi.data() + im.chi_off, im.points() * im.nlm * sizeof(double));
+    if (chi) {
+        const size_t n = (size_t)(im.points() * im.nlm);
+        if (!S.chi.empty()) memcpy(chi, S.chi.data() + im.chi_off, n * sizeof(double));
+        else {
+            HIP_TRY(hipSetDevice(c->device));
+            HIP_TRY(hipMemcpy(chi, c->chi64 + im.chi_off, n * sizeof(double), hipMemcpyDeviceToHost));
+        }
+    }
     return SQ_OK;
 }
 

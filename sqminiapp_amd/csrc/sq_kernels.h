@@ -73,6 +73,9 @@ hipError_t launch_density_energy(const double *lam, const double *w, long n_node
                                  double *out2, hipStream_t st);
 hipError_t launch_lam_range(const double *lmin, const double *lmax, long n, double *out, hipStream_t st);
 
+// FP64 -> FP32 copy of a table (sq_setup.hip)
+hipError_t launch_to_float(const double *a, float *b, long n, hipStream_t st);
+
 // ---- V_eff pipeline of the SCF (sq_scf.hip); fields on the periodic cell G
 struct StencilCoeffs {
     double w[3][11];   // -(1/2) Lap_h weights per axis, w[d][0] the centre

@@ -73,6 +73,7 @@ bool RadialSpline::eval(double r, double *out) const
     if (!(r >= x[0] && r <= x[n - 1])) return false;
     int j = (int)(std::lower_bound(x, x + n, r) - x);
     if (r < x[j]) j--;
+    if (j > n - 2) j = n - 2;   // r on the last knot: the last interval
     double p1 = x[j];
     double dx = 1.0 / (x[j + 1] - x[j]);
     double dy = (y[j + 1] - y[j]) * dx;
@@ -490,7 +491,7 @@ void collect_images(System &S)
 // chi_lm(x) = S_l(|x - R|) Y_lm(x - R) on each image box, stored with the box
 // point outer and channel inner (CalculateNonlocalProjectors,
 // lsdft_nl.cc:141-208, flattened as NonlocalProjectors, lsdft_nl.cc:81-123).
-Status fill_projectors(System &S)
+void layout_projectors(System &S)
 {
     int64_t total = 0;
     S.max_nlm = 0;
@@ -499,7 +500,7 @@ Status fill_projectors(System &S)
         total += im.points() * im.nlm;
         S.max_nlm = std::max(S.max_nlm, im.nlm);
     }
-    S.chi.assign((size_t)total, 0.0);
+    S.chi_total = total;
     int64_t rtotal = 0;
     for (Image &im : S.images) {
         const Pseudopotential &P = S.psd[im.type];
@@ -509,7 +510,13 @@ Status fill_projectors(System &S)
         im.rad_off = rtotal;
         rtotal += im.points() * im.nl;
     }
-    S.rad.assign((size_t)rtotal, 0.0);
+    S.rad_total = rtotal;
+}
+
+Status fill_projectors(System &S)
+{
+    S.chi.assign((size_t)S.chi_total, 0.0);
+    S.rad.assign((size_t)S.rad_total, 0.0);
     std::atomic<int> next{0}, bad{0};
     auto worker = [&]() {
         for (;;) {
@@ -859,7 +866,8 @@ Status build_system(const std::string &dir, const std::string &name, const Setup
         S.extn[d] = S.box[3 + d] + 2 * S.hw[d];
     }
     collect_images(S);
-    if (!(st = fill_projectors(S)).ok()) return st;
+    layout_projectors(S);
+    if (opt.host_tables && !(st = fill_projectors(S)).ok()) return st;
     build_coverage(S);
     return {};
 }

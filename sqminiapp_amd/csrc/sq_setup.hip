@@ -1,0 +1,226 @@
+// sq_setup.hip -- projector tables on the device: chi_lm = S_l(r) Y_lm and the
+// radial factors T_l of every image box point (CalculateNonlocalProjectors,
+// lsdft_nl.cc:141-208, flattened as NonlocalProjectors, lsdft_nl.cc:81-123),
+// one workgroup per image.  Every value is formed with the operations and
+// order of the host fill (sq_setup.cpp: RadialSpline::eval,
+// real_spherical_harmonic, fill_projectors) -- only +, -, *, / and sqrt, all
+// correctly rounded here -- so the tables are bit-identical to it.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "sq_system.h"
+
+namespace sqamd {
+
+namespace {
+
+constexpr int kTabThreads = 256;
+
+struct TabImage {
+    int lo[3], n[3];
+    double X[3];
+    int type, nlm, nl;
+    long long chi_off, rad_off;
+};
+
+struct TabType {
+    int r_off, n, lmax, lloc, nlm;
+    int y_off[4], yd_off[4];   // U_l dV_l values and not-a-knot slopes in the packed table
+    int ch_l[16], ch_m[16];
+};
+
+// SplineEvaluate on the interval holding r (first knot >= r, stepped back
+// unless r sits on it; the last knot takes the last interval).
+__device__ bool spline_at(const double *x, const double *y, const double *yd, int n, double r, double *out)
+{
+    if (!(r >= x[0] && r <= x[n - 1])) return false;
+    int lo = 0, hi = n;   // lower_bound
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (x[mid] < r) lo = mid + 1;
+        else hi = mid;
+    }
+    int j = lo;
+    if (r < x[j]) j--;
+    if (j > n - 2) j = n - 2;
+    const double p1 = x[j];
+    const double dx = 1.0 / (x[j + 1] - x[j]);
+    const double dy = (y[j + 1] - y[j]) * dx;
+    const double A0 = y[j], A1 = yd[j];
+    const double A2 = dx * (3.0 * dy - 2.0 * yd[j] - yd[j + 1]);
+    const double A3 = dx * dx * (-2.0 * dy + yd[j] + yd[j + 1]);
+    const double t = r - p1;
+    *out = ((A3 * t + A2) * t + A1) * t + A0;
+    return true;
+}
+
+// RealSphericalHarmonic for l <= 3 (lsdft_tools.cc:156-240), the reference's
+// 15-digit constants and expressions.
+__device__ double ylm(double x, double y, double z, int l, int m, bool *bad)
+{
+    const double c00 = 0.282094791773878;
+    const double c1 = 0.488602511902920;
+    const double c2a = 1.092548430592079, c20 = 0.315391565252520, c22 = 0.546274215296040;
+    const double c33 = 0.590043589926644, c32n = 2.890611442640554, c31 = 0.457045799464466;
+    const double c30 = 0.373176332590115, c32p = 1.445305721320277;
+    *bad = false;
+    const double r = sqrt(x * x + y * y + z * z);
+    if (l == 0) return c00;
+    if (l > 3 || m < -l || m > l) { *bad = true; return 0.0; }
+    if (!(r >= 1e-10)) return 0.0;
+    switch (l * 10 + (m + 3)) {
+    case 12: return c1 * (y / r);
+    case 13: return c1 * (z / r);
+    case 14: return c1 * (x / r);
+    case 21: return c2a * (x * y) / (r * r);
+    case 22: return c2a * (y * z) / (r * r);
+    case 23: return c20 * (-x * x - y * y + 2.0 * z * z) / (r * r);
+    case 24: return c2a * (z * x) / (r * r);
+    case 25: return c22 * (x * x - y * y) / (r * r);
+    case 30: return c33 * (3 * x * x - y * y) * y / (r * r * r);
+    case 31: return c32n * (x * y * z) / (r * r * r);
+    case 32: return c31 * y * (4 * z * z - x * x - y * y) / (r * r * r);
+    case 33: return c30 * z * (2 * z * z - 3 * x * x - 3 * y * y) / (r * r * r);
+    case 34: return c31 * x * (4 * z * z - x * x - y * y) / (r * r * r);
+    case 35: return c32p * z * (x * x - y * y) / (r * r * r);
+    case 36: return c33 * x * (x * x - 3 * y * y) / (r * r * r);
+    }
+    *bad = true;
+    return 0.0;
+}
+
+__global__ void __launch_bounds__(kTabThreads) projector_tables(const TabImage *__restrict__ imgs,
+                                                                const TabType *__restrict__ types,
+                                                                const double *__restrict__ tab, double h0, double h1,
+                                                                double h2, double R0, double R1, double R2,
+                                                                double *__restrict__ chi, double *__restrict__ rad,
+                                                                int *__restrict__ status)
+{
+    const TabImage im = imgs[blockIdx.x];
+    const TabType &T = types[im.type];
+    const double *rx = tab + T.r_off;
+    const long np = (long)im.n[0] * im.n[1] * im.n[2];
+    for (long p = threadIdx.x; p < np; p += blockDim.x) {
+        const int i = (int)(p % im.n[0]);
+        const int j = (int)((p / im.n[0]) % im.n[1]);
+        const int k = (int)(p / ((long)im.n[0] * im.n[1]));
+        const double z = h2 * (im.lo[2] + k) - R2 - im.X[2], zz = z * z;
+        const double y = h1 * (im.lo[1] + j) - R1 - im.X[1], yy = y * y;
+        const double x = h0 * (im.lo[0] + i) - R0 - im.X[0];
+        const double r = sqrt(x * x + yy + zz);
+        double radial[4] = {0.0, 0.0, 0.0, 0.0};
+        for (int l = 0; l <= T.lmax; l++) {
+            if (l == T.lloc) continue;
+            if (!spline_at(rx, tab + T.y_off[l], tab + T.yd_off[l], T.n, r, &radial[l])) { *status = 3; return; }
+        }
+        double *out = chi + im.chi_off + p * im.nlm;
+        for (int ch = 0; ch < im.nlm; ch++) {
+            bool b = false;
+            const double sh = ylm(x, y, z, T.ch_l[ch], T.ch_m[ch], &b);
+            if (b) { *status = 4; return; }
+            out[ch] = radial[T.ch_l[ch]] * sh;
+        }
+        double *rout = rad + im.rad_off + p * im.nl;
+        int q = 0;
+        for (int l = 0; l <= T.lmax; l++) {
+            if (l == T.lloc) continue;
+            double t;
+            if (l == 0) t = radial[0] * 0.282094791773878;
+            else if (!(r >= 1e-10)) t = 0.0;
+            else {
+                double rl = r;
+                for (int e = 1; e < l; e++) rl *= r;
+                t = radial[l] / rl;
+            }
+            rout[q++] = t;
+        }
+    }
+}
+
+__global__ void to_float(const double *__restrict__ a, float *__restrict__ b, long n)
+{
+    for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) b[q] = (float)a[q];
+}
+
+}  // namespace
+
+hipError_t launch_to_float(const double *a, float *b, long n, hipStream_t st)
+{
+    if (n <= 0) return hipSuccess;
+    const long blocks = std::min<long>(4096, (n + 255) / 256);
+    hipLaunchKernelGGL(to_float, dim3((unsigned)blocks), dim3(256), 0, st, a, b, n);
+    return hipGetLastError();
+}
+
+int build_projector_tables(const System &S, double *chi, double *rad, int *status, void *stream)
+{
+    hipStream_t st = (hipStream_t)stream;
+    if (S.images.empty()) return 0;
+    std::vector<double> tab;
+    std::vector<TabType> types(S.ntype);
+    for (int at = 0; at < S.ntype; at++) {
+        const Pseudopotential &P = S.psd[at];
+        TabType &T = types[at];
+        T.r_off = (int)tab.size();
+        T.n = P.size;
+        T.lmax = P.lmax;
+        T.lloc = P.lloc;
+        T.nlm = P.nlm();
+        if (T.nlm > 16) return 4;
+        tab.insert(tab.end(), P.r.begin(), P.r.end());
+        for (int l = 0; l < 4; l++) {
+            T.y_off[l] = T.yd_off[l] = 0;
+            if (l > P.lmax || l == P.lloc) continue;
+            T.y_off[l] = (int)tab.size();
+            tab.insert(tab.end(), P.s_udv[l].y.begin(), P.s_udv[l].y.end());
+            T.yd_off[l] = (int)tab.size();
+            tab.insert(tab.end(), P.s_udv[l].yd.begin(), P.s_udv[l].yd.end());
+        }
+        for (int c = 0; c < 16; c++) {
+            T.ch_l[c] = c < T.nlm ? P.ch_l[c] : 0;
+            T.ch_m[c] = c < T.nlm ? P.ch_m[c] : 0;
+        }
+    }
+    std::vector<TabImage> imgs(S.images.size());
+    for (size_t g = 0; g < S.images.size(); g++) {
+        const Image &src = S.images[g];
+        TabImage &im = imgs[g];
+        for (int d = 0; d < 3; d++) {
+            im.lo[d] = src.lo[d];
+            im.n[d] = src.dim(d);
+        }
+        im.X[0] = src.X0;
+        im.X[1] = src.Y0;
+        im.X[2] = src.Z0;
+        im.type = src.type;
+        im.nlm = src.nlm;
+        im.nl = src.nl;
+        im.chi_off = src.chi_off;
+        im.rad_off = src.rad_off;
+    }
+    TabImage *dimg = nullptr;
+    TabType *dtyp = nullptr;
+    double *dtab = nullptr;
+    int rc = 0;
+    if (hipMalloc((void **)&dimg, imgs.size() * sizeof(TabImage)) != hipSuccess ||
+        hipMalloc((void **)&dtyp, types.size() * sizeof(TabType)) != hipSuccess ||
+        hipMalloc((void **)&dtab, tab.size() * sizeof(double)) != hipSuccess)
+        rc = 6;
+    if (!rc && (hipMemcpyAsync(dimg, imgs.data(), imgs.size() * sizeof(TabImage), hipMemcpyHostToDevice, st) != hipSuccess ||
+                hipMemcpyAsync(dtyp, types.data(), types.size() * sizeof(TabType), hipMemcpyHostToDevice, st) != hipSuccess ||
+                hipMemcpyAsync(dtab, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice, st) != hipSuccess))
+        rc = 8;
+    if (!rc) {
+        hipLaunchKernelGGL(projector_tables, dim3((unsigned)imgs.size()), dim3(kTabThreads), 0, st, dimg, dtyp, dtab, S.h[0],
+                           S.h[1], S.h[2], S.range[0], S.range[1], S.range[2], chi, rad, status);
+        if (hipGetLastError() != hipSuccess || hipStreamSynchronize(st) != hipSuccess) rc = 8;
+    }
+    if (dimg) (void)hipFree(dimg);
+    if (dtyp) (void)hipFree(dtyp);
+    if (dtab) (void)hipFree(dtab);
+    return rc;
+}
+
+}  // namespace sqamd

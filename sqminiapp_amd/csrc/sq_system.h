@@ -92,6 +92,7 @@ struct System {
     // 0 at r < 1e-10), so chi_lm = T_l C_lm P_lm(x,y,z) with P_lm the
     // homogeneous polynomial of the real harmonic.
     std::vector<double> rad;
+    int64_t chi_total = 0, rad_total = 0;   // sizes of chi and rad (also when only the device holds them)
     // ---- per extended-box point: images whose box contains it, ascending
     std::vector<int32_t> cov_off, cov_img;
     int max_cov = 0, max_nlm = 0;
@@ -106,6 +107,7 @@ struct SetupOptions {
     double phantom_pos[3] = {0, 0, 0};
     int rank_box[6] = {0, 0, 0, 0, 0, 0};  // lx <= 0: whole grid
     int nqp_override = 0;
+    bool host_tables = true;   // false: chi and rad are left to the device (build_projector_tables)
 };
 
 // Reads the deck and builds every host table of the path.
@@ -118,6 +120,12 @@ Status build_system(const std::string &dir, const std::string &name, const Setup
 // densities scaled to N_e.
 Status build_pseudocharge(const System &S, std::vector<double> &b, double &eself, double &scale);
 Status build_atomic_density(const System &S, std::vector<double> &rho, double &scale);
+
+// chi and rad of every image on the device (sq_setup.hip), with the host
+// fill's arithmetic: the tables come out bit-identical.  chi/rad: device
+// arrays of S.chi_total / S.rad_total doubles; status: device int, set to
+// nonzero when a radius falls off a radial table.
+int build_projector_tables(const System &S, double *chi, double *rad, int *status, void *stream);
 
 // Pieces exposed for the reference-named host entry points.
 double real_spherical_harmonic(double x, double y, double z, int l, int m, bool *bad);
