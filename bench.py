@@ -86,7 +86,7 @@ def main():
     ap.add_argument("--config", default="C16")
     ap.add_argument("--mode", choices=["kb", "ref"], default="kb")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-sample", type=int, default=192)
+    ap.add_argument("--cpu-sample", type=int, default=512)
     ap.add_argument("--fp32-steps", type=int, default=None)
     ap.add_argument("--no-scf", action="store_true", help="skip the SCF wall-time measurement")
     ap.add_argument("--scf-iter", type=int, default=60)
