@@ -54,20 +54,28 @@ KERNEL_NAMES = {0: "lanczos_kernel (generic)", 1: "lanczos_zmarch (column-marchi
 def scf_wall_time(d, name, max_iter, mode):
     """SCF wall time on the periodic cell of the same deck: plain FP64 to a
     density residual of 1e-6, and the FP32 form to 1e-5, its energy judged
-    against FP64 (the metric's 1e-5 Ha/atom)."""
+    against FP64 (the metric's 1e-5 Ha/atom); then the forces of each form
+    (the metric's 1e-4 Ha/bohr), their time reported beside the SCF's."""
     import sqminiapp_amd as sq
     with sq.Context(d, name, device=0, cell=True) as c:
         I = c.info
         r64 = c.scf(mode, sq.SQ_FP64, tol=1e-6, max_iter=max_iter)
+        F64, _, _, fi64 = c.forces() if mode == sq.SQ_NL_KB else (None, None, None, None)
         r32 = c.scf(mode, sq.SQ_FP32, tol=1e-5, max_iter=max_iter)
+        F32, _, _, fi32 = c.forces() if mode == sq.SQ_NL_KB else (None, None, None, None)
     keep = ("iterations", "converged", "residual", "wall_s", "sq_s", "poisson_s", "e_total", "lambda_f")
     f64 = {k: r64[k] for k in keep}
     f32 = {k: r32[k] for k in keep}
-    return {"workload": f"periodic cell {I.rank_box[3]}x{I.rank_box[4]}x{I.rank_box[5]} = {I.n_nodes} nodes, "
-                        f"BETA {I.beta:g}, deck mixing (Periodic Pulay), LDA-PW, start: superposed atoms",
-            "plain_fp64": f64, "fp32_form": f32,
-            "d_energy_per_atom": abs(r32["e_total"] - r64["e_total"]) / I.nAtoms,
-            "wall_speedup_fp32": r64["wall_s"] / r32["wall_s"] if r32["wall_s"] > 0 else None}
+    out = {"workload": f"periodic cell {I.rank_box[3]}x{I.rank_box[4]}x{I.rank_box[5]} = {I.n_nodes} nodes, "
+                       f"BETA {I.beta:g}, deck mixing (Periodic Pulay), LDA-PW, start: superposed atoms",
+           "plain_fp64": f64, "fp32_form": f32,
+           "d_energy_per_atom": abs(r32["e_total"] - r64["e_total"]) / I.nAtoms,
+           "wall_speedup_fp32": r64["wall_s"] / r32["wall_s"] if r32["wall_s"] > 0 else None}
+    if F64 is not None:
+        out["forces"] = {"fp64_s": fi64["wall_s"], "fp32_s": fi32["wall_s"], "nodes": fi64["nodes"],
+                         "max_abs_fp64": float(abs(F64).max()), "max_diff_fp32_fp64": float(abs(F32 - F64).max()),
+                         "unit": "Ha/bohr"}
+    return out
 
 
 def load_traffic(tag):

@@ -245,6 +245,29 @@ int sq_xc(sq_ctx *ctx, const double *rho, long n, double *eps, double *vxc);
  * TOL_POISSON), mean(phi) = 0; f must be neutral up to rounding. */
 int sq_poisson(sq_ctx *ctx, const double *f, double tol, double *phi, int *iterations);
 
+/* ------------------------------------------------------------ forces
+ * Calculate_localforce and Calculate_nonlocalforce (declared, lsdft.h:188-189)
+ * with CalculateGradientDensityMatrixColumn / CalculateNonlocalForceContribution
+ * (declared, lsdft.h:224-226): the forces on the atoms (Ha/bohr) at the last SQ
+ * pass of sq_scf on this context, as the derivative of the SCF energy at that
+ * pass's input density.
+ *   local    : the pseudocharge term, from phi (host);
+ *   nonlocal : F_J = 4 sum_c sum_lm grad chi_J,lm(c) gamma <chi_J,lm, D_c>, D_c the
+ *              node's Gauss-quadrature density-matrix column Q f(T) e_1, from a
+ *              repeat of the pass over the nodes inside projector boxes.
+ * forces, f_local, f_nonlocal: 3 * nAtoms each (x, y, z per atom, .ion order);
+ * any may be NULL.  Needs the Kleinman-Bylander operator (SQ_NL_KB). */
+typedef struct {
+    double lambda_f;         /* Fermi level of the pass */
+    double wall_s;           /* whole call */
+    double nonlocal_s;       /* the repeated pass, Fermi columns and contraction */
+    long nodes;              /* nodes inside a projector box */
+    int slots;               /* most projector boxes holding one node */
+    int chunks;              /* node chunks the record was split into */
+} sq_force_info;
+
+int sq_forces(sq_ctx *ctx, double *forces, double *f_local, double *f_nonlocal, sq_force_info *info);
+
 #ifdef __cplusplus
 }
 #endif

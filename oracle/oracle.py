@@ -88,6 +88,11 @@ def lib():
         L.sqo_spline_eval.argtypes = [dp, dp, C.c_int, C.c_double, dp, dp]
         L.sqo_ylm.argtypes = [C.c_double, C.c_double, C.c_double, C.c_int, C.c_int, ip]
         L.sqo_ylm.restype = C.c_double
+        L.sqo_ylm_grad.argtypes = [C.c_double, C.c_double, C.c_double, C.c_int, C.c_int, dp]
+        L.sqo_chi_grad.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, dp, dp]
+        L.sqo_fermi_column.argtypes = [dp, dp, C.c_int, C.c_double, C.c_double, dp]
+        L.sqo_local_forces.argtypes = [vp, dp, C.c_double, dp]
+        L.sqo_nonlocal_forces.argtypes = [vp, dp, C.c_double, C.c_double, C.c_double, ip, C.c_int, C.c_int, dp, dp]
         assert L.sqo_info_size() == C.sizeof(Info), "oracle Info layout mismatch"
         _lib = L
     return _lib
@@ -146,6 +151,22 @@ def tridiag_nodes_weights(a, b, n):
     lmax = C.c_double()
     _check(lib().sqo_tridiag_nodes_weights(_dp(a), _dp(bb), n, _dp(lam), _dp(w), C.byref(lmin), C.byref(lmax)), "QL")
     return lam, w, lmin.value, lmax.value
+
+
+def ylm_grad(x, y, z, l, m):
+    g = np.zeros(3)
+    lib().sqo_ylm_grad(float(x), float(y), float(z), int(l), int(m), _dp(g))
+    return g
+
+
+def fermi_column(a, b, n, beta, lf):
+    """f(T) e_1 for the n x n tridiagonal of Lanczos coefficients a, b."""
+    a = np.ascontiguousarray(a[:n], np.float64)
+    bb = np.zeros(n, np.float64)
+    bb[: min(n, len(b))] = b[: min(n, len(b))]
+    c = np.zeros(n)
+    _check(lib().sqo_fermi_column(_dp(a), _dp(bb), n, float(beta), float(lf), _dp(c)), "fermi_column")
+    return c
 
 
 def density_fermi_energy(lam, w, beta, nelec, dV):
@@ -339,3 +360,34 @@ class System:
         _check(lib().sqo_nodes_and_weights(self._h, mode, _dp(field), const, _ip(ids), n, _dp(lam), _dp(w),
                                            _dp(lmin), _dp(lmax), int(nthreads)), "nodes_and_weights")
         return lam, w, lmin, lmax
+
+    # ---- forces (the derivative of the SCF energy; see sq_oracle.c)
+    def chi_grad(self, at, l, m, x, y, z):
+        """(chi_lm, grad chi_lm) of type at at the offset (x, y, z) from the atom."""
+        v = C.c_double()
+        g = np.zeros(3)
+        _check(lib().sqo_chi_grad(self._h, at, l, m, float(x), float(y), float(z), C.byref(v), _dp(g)), "chi_grad")
+        return v.value, g
+
+    def local_forces(self, phi, scale):
+        """Pseudocharge forces [nAtoms, 3] for the potential phi on the periodic cell."""
+        phi = np.ascontiguousarray(phi, np.float64).ravel()
+        F = np.zeros(3 * self.info().nAtoms)
+        _check(lib().sqo_local_forces(self._h, _dp(phi), float(scale), _dp(F)), "local_forces")
+        return F.reshape(-1, 3)
+
+    def nonlocal_forces(self, beta, lf, veff=None, node_ids=None, nthreads=1, per_node=False):
+        """Kleinman-Bylander forces [nAtoms, 3] at Fermi level lf (and each node's part)."""
+        I = self.info()
+        box = self.box
+        total = box[3] * box[4] * box[5]
+        ids = None if node_ids is None else np.ascontiguousarray(node_ids, np.int32)
+        n = total if ids is None else len(ids)
+        F = np.zeros(3 * I.nAtoms)
+        Fn = np.zeros(n * I.nAtoms * 3) if per_node else None
+        field, const = self._veff(veff)
+        _check(lib().sqo_nonlocal_forces(self._h, _dp(field), const, float(beta), float(lf), _ip(ids), n,
+                                         int(nthreads), _dp(F), _dp(Fn)), "nonlocal_forces")
+        if per_node:
+            return F.reshape(-1, 3), Fn.reshape(n, I.nAtoms, 3)
+        return F.reshape(-1, 3)

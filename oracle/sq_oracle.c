@@ -147,7 +147,17 @@ int sqo_spline_yd(const double *X, const double *Y, double *YD, int len)
  * if(p2<X1[j]) j--;"); the first X1[j] >= p2 is found here by bisection, which
  * selects the same j.  Out-of-range abscissae are an error (the reference
  * calls exit(1), lsdft_tools.cc:20-26). */
+static int spline_eval_d(const double *X1, const double *Y1, int len1, double p2, const double *YD, double *out,
+                         double *dout);
 int sqo_spline_eval(const double *X1, const double *Y1, int len1, double p2, const double *YD, double *out)
+{
+    return spline_eval_d(X1, Y1, len1, p2, YD, out, NULL);
+}
+
+/* The same cubic and, if dout is given, its derivative (3 A3 x + 2 A2) x + A1:
+ * the exact derivative of the interpolant the value comes from. */
+static int spline_eval_d(const double *X1, const double *Y1, int len1, double p2, const double *YD, double *out,
+                         double *dout)
 {
     if (p2 < X1[0] || p2 > X1[len1 - 1]) return SQO_ERR_RANGE;
     int lo = 0, hi = len1; /* first index with X1[idx] >= p2 */
@@ -166,6 +176,7 @@ int sqo_spline_eval(const double *X1, const double *Y1, int len1, double p2, con
     double A3 = dx * dx * (-2.0 * dy + YD[j] + YD[j + 1]);
     double x = p2 - p1;
     *out = ((A3 * x + A2) * x + A1) * x + A0;
+    if (dout) *dout = (3.0 * A3 * x + 2.0 * A2) * x + A1;
     return SQO_OK;
 }
 
@@ -216,18 +227,14 @@ double sqo_ylm(double x, double y, double z, int l, int m, int *err)
 /* TridiagEigenVecSolve_NodesAndWeights, lsdft_tools.cc:912-1026: implicit QL
  * (Numerical Recipes tqli) with the full eigenvector matrix z (z = I at the
  * start); lambda_i = d_i (unsorted), w_i = z[0][i]^2. */
-int sqo_tridiag_nodes_weights(const double *diag, const double *subdiag, int n,
-                              double *lambda, double *w, double *lmin, double *lmax)
+/* The QL iteration itself: d (diagonal, n), e (subdiagonal, n; e[n-1] is
+ * overwritten), z (n*n, the identity on entry) -> eigenvalues in d,
+ * eigenvectors in the columns of z. */
+static int ql_full(double *d, double *e, double *z, int n)
 {
     int m, l, iter, i, k;
     double s, r, p, g, f, dd, c, b;
-    double *d = dalloc(n), *e = dalloc(n), *z = dalloc((size_t)n * n);
 #define Z(a, bb) z[(size_t)(a) * n + (bb)]
-    for (i = 0; i < n; i++) {
-        d[i] = diag[i];
-        e[i] = subdiag[i];
-        Z(i, i) = 1.0;
-    }
     e[n - 1] = 0.0;
     for (l = 0; l <= n - 1; l++) {
         iter = 0;
@@ -237,7 +244,7 @@ int sqo_tridiag_nodes_weights(const double *diag, const double *subdiag, int n,
                 if ((double)(fabs(e[m]) + dd) == dd) break;
             }
             if (m != l) {
-                if (iter++ == 50) { free(d); free(e); free(z); return SQO_ERR_QL; }
+                if (iter++ == 50) return SQO_ERR_QL;
                 g = (d[l + 1] - d[l]) / (2.0 * e[l]);
                 r = sqrt(g * g + 1.0);
                 g = d[m] - d[l] + e[l] / (g + (g >= 0.0 ? fabs(r) : -fabs(r)));
@@ -271,16 +278,30 @@ int sqo_tridiag_nodes_weights(const double *diag, const double *subdiag, int n,
             }
         } while (m != l);
     }
+#undef Z
+    return SQO_OK;
+}
+
+int sqo_tridiag_nodes_weights(const double *diag, const double *subdiag, int n,
+                              double *lambda, double *w, double *lmin, double *lmax)
+{
+    double *d = dalloc(n), *e = dalloc(n), *z = dalloc((size_t)n * n);
+    for (int i = 0; i < n; i++) {
+        d[i] = diag[i];
+        e[i] = subdiag[i];
+        z[(size_t)i * n + i] = 1.0;
+    }
+    int rc = ql_full(d, e, z, n);
+    if (rc) { free(d); free(e); free(z); return rc; }
     double emin = d[0], emax = d[0];
-    for (i = 0; i < n; i++) {
+    for (int i = 0; i < n; i++) {
         lambda[i] = d[i];
-        w[i] = Z(0, i) * Z(0, i);
+        w[i] = z[i] * z[i];   /* z[0][i]^2 */
         if (d[i] > emax) emax = d[i];
         else if (d[i] < emin) emin = d[i];
     }
     if (lmin) *lmin = emin;
     if (lmax) *lmax = emax;
-#undef Z
     free(d); free(e); free(z);
     return SQO_OK;
 }
@@ -959,7 +980,7 @@ static void spmv(const sqo_sys *S, const double *dval, const double *x, double *
  * mode 1 = Mult_HamiltonianVector (lsdft_matvec.cc:155-293): slot
  *          IP_index[p] + (l,m) channel, the Kleinman-Bylander operator.
  * Entries are scanned over every image of the rank box, with the cube test of
- * lsdft_matvec.cc:95,134. */
+ * lsdft_matvec.cc:95,134.  V2 == NULL computes alpha only (the first pass). */
 static void nonlocal_apply(const sqo_sys *S, int mode, double *alpha, int Kp, int Jp, int Ip,
                            const double *V1, double *V2)
 {
@@ -967,7 +988,7 @@ static void nonlocal_apply(const sqo_sys *S, int mode, double *alpha, int Kp, in
     int hx = (int)ceil(S->rcut / S->delta[0]), hy = (int)ceil(S->rcut / S->delta[1]), hz = (int)ceil(S->rcut / S->delta[2]);
     int Xstart = Ip - hx, Xend = Ip + hx, Ystart = Jp - hy, Yend = Jp + hy, Zstart = Kp - hz, Zend = Kp + hz;
     memset(alpha, 0, sizeof(double) * (S->IP_length > 0 ? S->IP_length : 1));
-    for (int pass = 0; pass < 2; pass++) {
+    for (int pass = 0; pass < (V2 ? 2 : 1); pass++) {
         for (int at = 0; at < S->ntype; at++) {
             const sqo_overlap *O = &S->ovl[at];
             const double *G = S->psd[at].Gamma;
@@ -1038,14 +1059,17 @@ static double vnorm(const double *x, int n)
 }
 
 /* CalculateSpectralNodesAndWeights, lsdft_scf.cc:87-164 (Lanczos part).
- * a[0..nqp], b[0..nqp] are written (a[nqp], b[nqp] are not used by the QL). */
-static int lanczos_node(const sqo_sys *S, int mode, node_ws *W, int Kp, int Jp, int Ip, double *a, double *b)
+ * a[0..nqp], b[0..nqp] are written (a[nqp], b[nqp] are not used by the QL).
+ * Q, if given, receives the Lanczos vectors q_0 .. q_{nqp-1} (nqp rows of nc). */
+static int lanczos_node(const sqo_sys *S, int mode, node_ws *W, int Kp, int Jp, int Ip, double *a, double *b,
+                        double *Q)
 {
     int n = S->nc, N_qp = S->nqp;
     memset(W->vk, 0, sizeof(double) * n);
     memset(W->vkm1, 0, sizeof(double) * n);
     memset(W->vkp1, 0, sizeof(double) * n);
     W->vkm1[S->center] = 1.0;
+    if (Q) memcpy(Q, W->vkm1, sizeof(double) * n);
     spmv(S, W->dval, W->vkm1, W->vk);
     nonlocal_apply(S, mode, W->alpha, Kp, Jp, Ip, W->vkm1, W->vk);
     a[0] = vdot(W->vkm1, W->vk, n);
@@ -1054,6 +1078,7 @@ static int lanczos_node(const sqo_sys *S, int mode, node_ws *W, int Kp, int Jp, 
     if (b[0] == 0.0) return SQO_ERR_BREAKDOWN;
     double s = 1.0 / b[0];
     for (int i = 0; i < n; i++) W->vk[i] = s * W->vk[i];
+    if (Q && N_qp > 1) memcpy(Q + n, W->vk, sizeof(double) * n);
     for (int k = 0; k < N_qp; k++) {
         spmv(S, W->dval, W->vk, W->vkp1);
         nonlocal_apply(S, mode, W->alpha, Kp, Jp, Ip, W->vk, W->vkp1);
@@ -1066,6 +1091,7 @@ static int lanczos_node(const sqo_sys *S, int mode, node_ws *W, int Kp, int Jp, 
         if (b[k + 1] == 0.0) return SQO_ERR_BREAKDOWN;
         s = 1.0 / b[k + 1];
         for (int i = 0; i < n; i++) W->vk[i] = s * W->vk[i];
+        if (Q && k + 2 < N_qp) memcpy(Q + (size_t)(k + 2) * n, W->vk, sizeof(double) * n);
     }
     return SQO_OK;
 }
@@ -1080,7 +1106,7 @@ int sqo_lanczos(sqo_sys *S, int mode, const double *veff_field, double veff_cons
     if (ws_init(&W, S)) { ws_free(&W); return SQO_ERR_ALLOC; }
     cube_veff(S, veff_field, veff_const, Kp, Jp, Ip, W.veff);
     rebuild_operator(S, &W);
-    int rc = lanczos_node(S, mode, &W, Kp, Jp, Ip, a, b);
+    int rc = lanczos_node(S, mode, &W, Kp, Jp, Ip, a, b, NULL);
     ws_free(&W);
     return rc;
 }
@@ -1114,7 +1140,7 @@ int sqo_nodes_and_weights(sqo_sys *S, int mode, const double *veff_field, double
             int i = S->box[0] + id % lx, j = S->box[1] + (id / lx) % ly, k = S->box[2] + id / (lx * ly);
             cube_veff(S, veff_field, veff_const, k, j, i, W.veff);
             rebuild_operator(S, &W);
-            int rc = lanczos_node(S, mode, &W, k, j, i, a, b);
+            int rc = lanczos_node(S, mode, &W, k, j, i, a, b, NULL);
             if (!rc) rc = sqo_tridiag_nodes_weights(a, b, S->nqp, lam + (size_t)r * S->nqp, w + (size_t)r * S->nqp,
                                                     lmin + r, lmax + r);
             if (rc) e = rc;
@@ -1387,6 +1413,275 @@ int sqo_atomic_density(sqo_sys *S, double *rho, double *scale)
     for (size_t q = 0; q < ng; q++) rho[q] *= sc;
     *scale = sc;
     return SQO_OK;
+}
+
+/* ------------------------------------------------------------ forces
+ * The reference declares its force routines only (Calculate_localforce,
+ * Calculate_nonlocalforce, CalculateGradientDensityMatrixColumn,
+ * CalculateNonlocalForceContribution, lsdft.h:188-191,223-227); it builds the
+ * projectors on the force overlaps (CalculateNonlocalProjectorsForces,
+ * lsdft_initObjs.cc:1044-1114) and the FD gradient weights
+ * (lsdft_readfiles.cc:490-497).  Written here as the derivative, with respect
+ * to an atom's position R_J, of the energy the SCF reports,
+ *   E = E_band + E_xc - int V_xc rho + (1/2) int phi (b - rho) - E_self - TS,
+ * at self-consistency (Hellmann-Feynman: only the explicit R dependence of b,
+ * E_self and V_nl counts; the density terms are stationary):
+ *
+ * local    F_J = f dV sum_{J'} (B g_J')^T phi  -  (f dV / 2) [(B g_s)^T V_s + b_s^T g_s]
+ *          with B = -(1/4pi) Lap_h (the pseudocharge stencil), g = grad V_loc
+ *          (analytic: the V_loc spline's derivative, Z/r^2 past the table)
+ *          sampled on each image box J' of atom J (the part inside the cell,
+ *          as b is placed), and the self term over the atom's own whole box
+ *          (b_s, V_s unscaled, f the neutrality scale of b);
+ * nonlocal F_J = 4 sum_c sum_{J' of J, c in box J'} sum_lm grad chi_J'lm(c) *
+ *                gamma_l <chi_J,lm, D_c>
+ *          from E_nl = 2 tr(V_nl D), V_nl = sum gamma chi chi^T (gamma carries
+ *          dV), the gradient taken on the node side of the symmetric D; D_c =
+ *          Q f(T) e_1 is the node's Gauss-quadrature density-matrix column (Q
+ *          the Lanczos vectors, T the tridiagonal of its rule), so
+ *          <chi, D_c>_c = rho_c dV / 2 for chi = e_c.  chi_J sums every image
+ *          of atom J in the cube, as the Hamiltonian's alpha does. */
+
+/* V_loc'(r) (x, y, z)/r; zero at r = 0. */
+static int vloc_grad(const sqo_sys *S, int at, double x, double y, double z, double g[3])
+{
+    const sqo_psd *P = &S->psd[at];
+    double r = sqrt(x * x + y * y + z * z), v, dv;
+    if (r > P->r[P->size - 1]) dv = (double)S->noe[at] / (r * r);
+    else if (spline_eval_d(P->r, P->Vloc, P->size, r, P->YD_Vloc, &v, &dv)) return SQO_ERR_RANGE;
+    if (r < 1e-10) { g[0] = g[1] = g[2] = 0.0; return SQO_OK; }
+    g[0] = dv * x / r; g[1] = dv * y / r; g[2] = dv * z / r;
+    return SQO_OK;
+}
+
+/* Gradient of the real harmonic C P_lm(x,y,z) / r^l of sqo_ylm:
+ * C (grad P / r^l - l P x / r^(l+2)); zero for r < 1e-10 (where the
+ * reference sets Y_lm = 0 for l > 0). */
+static void ylm_grad(double x, double y, double z, int l, int m, double g[3])
+{
+    double r2 = x * x + y * y + z * z, r = sqrt(r2);
+    g[0] = g[1] = g[2] = 0.0;
+    if (l == 0 || r < 1e-10) return;
+    double C = 0.0, P = 0.0, d[3] = {0.0, 0.0, 0.0};
+    double xx = x * x, yy = y * y, zz = z * z;
+    switch (l * 10 + m + 3) {
+    case 12: C = 0.488602511902920; P = y; d[1] = 1; break;
+    case 13: C = 0.488602511902920; P = z; d[2] = 1; break;
+    case 14: C = 0.488602511902920; P = x; d[0] = 1; break;
+    case 21: C = 1.092548430592079; P = x * y; d[0] = y; d[1] = x; break;
+    case 22: C = 1.092548430592079; P = y * z; d[1] = z; d[2] = y; break;
+    case 23: C = 0.315391565252520; P = -xx - yy + 2 * zz; d[0] = -2 * x; d[1] = -2 * y; d[2] = 4 * z; break;
+    case 24: C = 1.092548430592079; P = z * x; d[0] = z; d[2] = x; break;
+    case 25: C = 0.546274215296040; P = xx - yy; d[0] = 2 * x; d[1] = -2 * y; break;
+    case 30: C = 0.590043589926644; P = (3 * xx - yy) * y; d[0] = 6 * x * y; d[1] = 3 * xx - 3 * yy; break;
+    case 31: C = 2.890611442640554; P = x * y * z; d[0] = y * z; d[1] = x * z; d[2] = x * y; break;
+    case 32: C = 0.457045799464466; P = y * (4 * zz - xx - yy); d[0] = -2 * x * y; d[1] = 4 * zz - xx - 3 * yy; d[2] = 8 * y * z; break;
+    case 33: C = 0.373176332590115; P = z * (2 * zz - 3 * xx - 3 * yy); d[0] = -6 * x * z; d[1] = -6 * y * z; d[2] = 6 * zz - 3 * xx - 3 * yy; break;
+    case 34: C = 0.457045799464466; P = x * (4 * zz - xx - yy); d[0] = 4 * zz - 3 * xx - yy; d[1] = -2 * x * y; d[2] = 8 * x * z; break;
+    case 35: C = 1.445305721320277; P = z * (xx - yy); d[0] = 2 * x * z; d[1] = -2 * y * z; d[2] = xx - yy; break;
+    case 36: C = 0.590043589926644; P = x * (xx - 3 * yy); d[0] = 3 * xx - 3 * yy; d[1] = -6 * x * y; break;
+    default: return;
+    }
+    double rl = 1.0;
+    for (int k = 0; k < l; k++) rl *= r;
+    double X[3] = {x, y, z};
+    for (int k = 0; k < 3; k++) g[k] = C * (d[k] / rl - l * P * X[k] / (rl * r2));
+}
+
+void sqo_ylm_grad(double x, double y, double z, int l, int m, double *g) { ylm_grad(x, y, z, l, m, g); }
+
+/* chi_lm = U_l dV_l(r) Y_lm at (x, y, z) from the atom and its gradient
+ * S'(r) (x/r) Y + S grad Y (zero at r < 1e-10). */
+int sqo_chi_grad(sqo_sys *S, int at, int l, int m, double x, double y, double z, double *val, double *g)
+{
+    const sqo_psd *P = &S->psd[at];
+    double r = sqrt(x * x + y * y + z * z), u, du, gy[3];
+    int e2 = 0;
+    if (spline_eval_d(P->r, P->UDeltaV[l], P->size, r, P->YD_UDeltaV[l], &u, &du)) return SQO_ERR_RANGE;
+    double Y = sqo_ylm(x, y, z, l, m, &e2);
+    if (e2) return e2;
+    *val = u * Y;
+    g[0] = g[1] = g[2] = 0.0;
+    if (r < 1e-10) return SQO_OK;
+    ylm_grad(x, y, z, l, m, gy);
+    double X[3] = {x, y, z};
+    for (int k = 0; k < 3; k++) g[k] = du * (X[k] / r) * Y + u * gy[k];
+    return SQO_OK;
+}
+
+/* c = f(T) e_1 = Z f(Lambda) Z^T e_1 for the n x n tridiagonal (diag, subdiag)
+ * with f(l) = 1/(1 + exp(beta (l - lf))): column 0 of the Gauss-quadrature
+ * density matrix in the Lanczos basis. */
+int sqo_fermi_column(const double *diag, const double *subdiag, int n, double beta, double lf, double *c)
+{
+    double *d = dalloc(n), *e = dalloc(n), *z = dalloc((size_t)n * n);
+    for (int i = 0; i < n; i++) { d[i] = diag[i]; e[i] = subdiag[i]; z[(size_t)i * n + i] = 1.0; }
+    int rc = ql_full(d, e, z, n);
+    if (!rc)
+        for (int k = 0; k < n; k++) {
+            double s = 0.0;
+            for (int j = 0; j < n; j++) s += z[(size_t)k * n + j] * fermi(beta * (d[j] - lf)) * z[j];
+            c[k] = s;
+        }
+    free(d); free(e); free(z);
+    return rc;
+}
+
+/* Local (pseudocharge) forces, F[3*nAtoms] (formula above).  phi is the
+ * electrostatic potential on the periodic cell, scale the factor f of b. */
+int sqo_local_forces(sqo_sys *S, const double *phi, double scale, double *F)
+{
+    const int G[3] = {S->npts[0] - 1, S->npts[1] - 1, S->npts[2] - 1};
+    const int o = S->order;
+    const double *scx = S->stencil_coeffs[0], *scy = S->stencil_coeffs[1], *scz = S->stencil_coeffs[2];
+    memset(F, 0, sizeof(double) * 3 * S->nAtoms);
+    for (int at = 0; at < S->ntype; at++) {
+        int Imax[3];
+        image_range(S, at, Imax);
+        for (int p = S->startPos[at] / 3; p <= S->endPos[at] / 3; p++) {
+            const double *A = &S->atompos[3 * p];
+            double fphi[3] = {0, 0, 0}, fself[3] = {0, 0, 0};
+            for (int PP = -Imax[0]; PP <= Imax[0]; PP++)
+                for (int QQ = -Imax[1]; QQ <= Imax[1]; QQ++)
+                    for (int RR = -Imax[2]; RR <= Imax[2]; RR++) {
+                        double X[3] = {A[0] + PP * 2.0 * S->range[0], A[1] + QQ * 2.0 * S->range[1],
+                                       A[2] + RR * 2.0 * S->range[2]};
+                        int lo[3], hi[3], clo[3], chi[3], empty = 0, n[3], e[3];
+                        psc_box(S, at, X, lo, hi);
+                        for (int d = 0; d < 3; d++) {
+                            clo[d] = lo[d] > 0 ? lo[d] : 0;
+                            chi[d] = hi[d] < G[d] - 1 ? hi[d] : G[d] - 1;
+                            empty |= clo[d] > chi[d];
+                            n[d] = hi[d] - lo[d] + 1;
+                            e[d] = n[d] + 2 * o;
+                        }
+                        const int self = PP == 0 && QQ == 0 && RR == 0;
+                        if (empty && !self) continue;
+                        size_t ne = (size_t)e[0] * e[1] * e[2], nb = (size_t)n[0] * n[1] * n[2];
+                        double *g = dalloc(3 * ne), *bj = NULL, *vj = NULL;
+                        for (int k = 0; k < e[2]; k++)
+                            for (int j = 0; j < e[1]; j++)
+                                for (int i = 0; i < e[0]; i++) {
+                                    double x = S->delta[0] * (lo[0] - o + i) - S->range[0] - X[0];
+                                    double y = S->delta[1] * (lo[1] - o + j) - S->range[1] - X[1];
+                                    double z = S->delta[2] * (lo[2] - o + k) - S->range[2] - X[2];
+                                    size_t q = ((size_t)k * e[1] + j) * e[0] + i;
+                                    if (vloc_grad(S, at, x, y, z, g + 3 * q)) { free(g); return SQO_ERR_RANGE; }
+                                }
+                        if (self) {
+                            bj = dalloc(nb); vj = dalloc(nb);
+                            if (psc_image(S, at, X, lo, hi, bj, vj)) { free(g); free(bj); free(vj); return SQO_ERR_RANGE; }
+                        }
+#define GG(k, j, i, d) g[3 * (((size_t)(k) * e[1] + (j)) * e[0] + (i)) + (d)]
+                        for (int k = o; k < n[2] + o; k++)
+                            for (int j = o; j < n[1] + o; j++)
+                                for (int i = o; i < n[0] + o; i++) {
+                                    int gi = lo[0] - o + i, gj = lo[1] - o + j, gk = lo[2] - o + k;
+                                    int inside = !empty && gi >= clo[0] && gi <= chi[0] && gj >= clo[1] && gj <= chi[1] &&
+                                                 gk >= clo[2] && gk <= chi[2];
+                                    if (!inside && !self) continue;
+                                    size_t q = ((size_t)(k - o) * n[1] + (j - o)) * n[0] + (i - o);
+                                    for (int d = 0; d < 3; d++) {
+                                        double v = GG(k, j, i, d) * (scx[0] + scy[0] + scz[0]);
+                                        for (int a = 1; a <= o; a++)
+                                            v += (GG(k, j, i - a, d) + GG(k, j, i + a, d)) * scx[a] +
+                                                 (GG(k, j - a, i, d) + GG(k, j + a, i, d)) * scy[a] +
+                                                 (GG(k - a, j, i, d) + GG(k + a, j, i, d)) * scz[a];
+                                        if (inside) fphi[d] += v * phi[((size_t)gk * G[1] + gj) * G[0] + gi];
+                                        if (self) fself[d] += v * vj[q] + bj[q] * GG(k, j, i, d);
+                                    }
+                                }
+#undef GG
+                        free(g); free(bj); free(vj);
+                    }
+            for (int d = 0; d < 3; d++) F[3 * p + d] = scale * S->dV * fphi[d] - 0.5 * scale * S->dV * fself[d];
+        }
+    }
+    return SQO_OK;
+}
+
+/* Nonlocal forces, F[3*nAtoms] (formula above), KB operator, from the nodes
+ * node_ids (NULL: every node of the rank box) with V_eff as for
+ * sqo_nodes_and_weights.  Fnode (optional) receives each node's
+ * contribution [n_nodes][nAtoms][3]; F sums them in node order. */
+int sqo_nonlocal_forces(sqo_sys *S, const double *veff_field, double veff_const, double beta, double lf,
+                        const int *node_ids, int n_nodes, int nthreads, double *F, double *Fnode)
+{
+    if (!S->have_nl) return SQO_ERR_ARG;
+    build_cube_laplacian(S);
+    int lx = S->box[3], ly = S->box[4], lz = S->box[5];
+    int total = lx * ly * lz, na = S->nAtoms, nq = S->nqp, nc = S->nc;
+    if (!node_ids) n_nodes = total;
+    double *own = NULL;
+    if (!Fnode) own = Fnode = dalloc((size_t)n_nodes * na * 3);
+    else memset(Fnode, 0, sizeof(double) * (size_t)n_nodes * na * 3);
+    int err = SQO_OK;
+    if (nthreads < 1) nthreads = 1;
+#pragma omp parallel num_threads(nthreads)
+    {
+        node_ws W;
+        int e = ws_init(&W, S);
+        double *a = dalloc(nq + 1), *b = dalloc(nq + 1), *cf = dalloc(nq), *Q = dalloc((size_t)nq * nc), *dcol = dalloc(nc);
+#pragma omp for schedule(dynamic, 1)
+        for (int r = 0; r < n_nodes; r++) {
+            if (e) continue;
+            int id = node_ids ? node_ids[r] : r;
+            if (id < 0 || id >= total) { e = SQO_ERR_ARG; continue; }
+            int I = S->box[0] + id % lx, J = S->box[1] + (id / lx) % ly, K = S->box[2] + id / (lx * ly);
+            /* images whose box holds the node */
+            int near = 0;
+            for (int at = 0; at < S->ntype && !near; at++)
+                for (int c = 0; c < S->ovl[at].n && !near; c++) {
+                    const sqo_overlap *O = &S->ovl[at];
+                    near = I >= O->xs[c] && I <= O->xe[c] && J >= O->ys[c] && J <= O->ye[c] && K >= O->zs[c] && K <= O->ze[c];
+                }
+            if (!near) continue;
+            cube_veff(S, veff_field, veff_const, K, J, I, W.veff);
+            rebuild_operator(S, &W);
+            int rc = lanczos_node(S, 1, &W, K, J, I, a, b, Q);
+            if (!rc) rc = sqo_fermi_column(a, b, nq, beta, lf, cf);
+            if (rc) { e = rc; continue; }
+            for (int i = 0; i < nc; i++) {
+                double s = 0.0;
+                for (int k = 0; k < nq; k++) s += cf[k] * Q[(size_t)k * nc + i];
+                dcol[i] = s;
+            }
+            nonlocal_apply(S, 1, W.alpha, K, J, I, dcol, NULL);   /* alpha = gamma <chi_atom, D_c> */
+            double *fn = Fnode + (size_t)r * na * 3;
+
+            for (int at = 0; at < S->ntype; at++) {
+                const sqo_overlap *O = &S->ovl[at];
+                int lloc = S->localPsd[at];
+                for (int c = 0; c < O->n; c++) {
+                    if (!(I >= O->xs[c] && I <= O->xe[c] && J >= O->ys[c] && J <= O->ye[c] && K >= O->zs[c] && K <= O->ze[c]))
+                        continue;
+                    int p = O->xindex[c] / 3, slot = S->IP_index[p];
+                    double x = S->delta[0] * I - S->range[0] - O->X0[c];
+                    double y = S->delta[1] * J - S->range[1] - O->Y0[c];
+                    double z = S->delta[2] * K - S->range[2] - O->Z0[c];
+                    for (int l = 0; l <= S->psd[at].lmax; l++) {
+                        if (l == lloc) continue;
+                        for (int m = -l; m <= l; m++, slot++) {
+                            double v, g[3];
+                            if (sqo_chi_grad(S, at, l, m, x, y, z, &v, g)) { e = SQO_ERR_RANGE; continue; }
+                            for (int d = 0; d < 3; d++) fn[3 * p + d] += 4.0 * g[d] * W.alpha[slot];
+                        }
+                    }
+                }
+            }
+        }
+        if (e) {
+#pragma omp critical
+            err = e;
+        }
+        free(a); free(b); free(cf); free(Q); free(dcol);
+        ws_free(&W);
+    }
+    memset(F, 0, sizeof(double) * 3 * na);
+    for (int r = 0; r < n_nodes; r++)
+        for (int q = 0; q < 3 * na; q++) F[q] += Fnode[(size_t)r * na * 3 + q];
+    free(own);
+    return err;
 }
 
 void sqo_info(sqo_sys *S, sqo_info_t *I)

@@ -26,7 +26,7 @@ EXPORTS = ["sq_last_error", "sq_default_options", "sq_create_from_deck", "sq_des
            "sq_run", "sq_get_nodes_and_weights", "sq_get_lanczos", "sq_nodes_and_weights", "sq_last_timing",
            "sq_apply_h", "sq_apply_nonlocal", "sq_tridiag_nodes_weights", "sq_density_fermi_energy", "sq_synchronize",
            "sq_get_image", "sq_default_scf_options", "sq_scf", "sq_get_cell_field", "sq_set_veff_cell", "sq_xc",
-           "sq_poisson", "sq_set_early_exit", "sq_get_rule_sizes"]
+           "sq_poisson", "sq_set_early_exit", "sq_get_rule_sizes", "sq_forces"]
 
 FIELDS = {"rho": 0, "rho_out": 1, "phi": 2, "vxc": 3, "veff": 4, "b": 5, "rho_atomic": 6}
 
@@ -55,6 +55,11 @@ class ScfResult(C.Structure):
                 ("minus_ts", C.c_double), ("e_xc", C.c_double), ("e_vxc", C.c_double), ("e_el", C.c_double),
                 ("e_self", C.c_double), ("e_total", C.c_double), ("wall_s", C.c_double), ("sq_s", C.c_double),
                 ("poisson_s", C.c_double), ("mean_rule_size", C.c_double)]
+
+
+class ForceInfo(C.Structure):
+    _fields_ = [("lambda_f", C.c_double), ("wall_s", C.c_double), ("nonlocal_s", C.c_double), ("nodes", C.c_long),
+                ("slots", C.c_int), ("chunks", C.c_int)]
 
 
 def deck_grid(directory, name):
@@ -110,6 +115,7 @@ def lib():
     L.sq_poisson.argtypes = [vp, dp, C.c_double, dp, C.POINTER(C.c_int)]
     L.sq_set_early_exit.argtypes = [vp, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int]
     L.sq_get_rule_sizes.argtypes = [vp, ip]
+    L.sq_forces.argtypes = [vp, dp, dp, dp, C.POINTER(ForceInfo)]
     L.sq_tridiag_nodes_weights.argtypes = [vp, dp, dp, C.c_int, lg, dp, dp, dp, dp]
     L.sq_density_fermi_energy.argtypes = [vp, C.c_double, C.c_double, dp, dp, dp, dp]
     L.sq_synchronize.argtypes = [vp]
@@ -276,6 +282,15 @@ class Context:
         r = ScfResult()
         _check(lib().sq_scf(self._h, C.byref(o), C.byref(r)))
         return {k: getattr(r, k) for k, _ in ScfResult._fields_}
+
+    def forces(self):
+        """Forces [nAtoms, 3] (Ha/bohr) at the last SCF pass: (total, local, nonlocal, info)."""
+        n = 3 * self.info.nAtoms
+        F, Fl, Fn = np.zeros(n), np.zeros(n), np.zeros(n)
+        fi = ForceInfo()
+        _check(lib().sq_forces(self._h, _dp(F), _dp(Fl), _dp(Fn), C.byref(fi)))
+        info = {k: getattr(fi, k) for k, _ in ForceInfo._fields_}
+        return F.reshape(-1, 3), Fl.reshape(-1, 3), Fn.reshape(-1, 3), info
 
     def cell_field(self, name):
         out = np.zeros(int(np.prod(self.cell_shape())))

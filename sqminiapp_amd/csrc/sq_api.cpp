@@ -14,6 +14,7 @@
 
 #include "../../include/sq_amd.h"
 #include "sq_kernels.h"
+#include "sq_projectors.h"
 #include "sq_system.h"
 
 using namespace sqamd;
@@ -95,6 +96,13 @@ struct sq_ctx {
         bool have_b = false;
         double eself = 0, b_scale = 0;
         std::vector<double> host_b;
+        // the last SQ pass of the loop, which sq_forces repeats: its input
+        // density's V_eff (left in the diagonal), phi, Fermi level and settings
+        bool force_ready = false;
+        double lambda_f = 0;
+        int last_mode = 1, last_prec = 0;
+        double last_exit_tol = 0, last_exit_lf = 0, last_exit_beta = 0;
+        int last_exit_min = 0, last_exit_every = 0;
     } scf;
 };
 
@@ -1109,6 +1117,12 @@ int sq_scf(sq_ctx *c, const sq_scf_options *opt_in, sq_scf_result *res)
         if (opt.exit_tol > 0 && it > 1)
             sq_set_early_exit(c, opt.exit_tol, res->lambda_f, 0.0, opt.exit_min > 0 ? opt.exit_min : 30,
                               opt.exit_every > 0 ? opt.exit_every : 5);
+        F.force_ready = false;
+        F.last_exit_tol = c->exit_tol;
+        F.last_exit_lf = c->exit_lf;
+        F.last_exit_beta = c->exit_beta;
+        F.last_exit_min = c->exit_min;
+        F.last_exit_every = c->exit_every;
         rc = sq_run(c, opt.nonlocal_mode, opt.precision, 0, n);
         c->exit_tol = 0.0;
         if (rc) return rc;
@@ -1122,6 +1136,10 @@ int sq_scf(sq_ctx *c, const sq_scf_options *opt_in, sq_scf_result *res)
         double lf, eb, mts;
         if ((rc = density_device(c, 0.0, 0.0, &lf, F.rho_out, &eb, &mts))) return rc;
         t_sq += std::chrono::duration<double>(std::chrono::steady_clock::now() - ts).count();
+        F.lambda_f = lf;
+        F.last_mode = opt.nonlocal_mode;
+        F.last_prec = opt.precision;
+        F.force_ready = true;
         // energies from the input density
         double exc, evxc, pb, prho, rr, oo;
         if ((rc = scf_dot(c, F.rho, F.eps, 6, &exc)) || (rc = scf_dot(c, F.rho, F.vxc, 7, &evxc)) ||
@@ -1213,6 +1231,200 @@ int sq_get_cell_field(sq_ctx *c, int which, double *out)
     HIP_TRY(hipSetDevice(c->device));
     HIP_TRY(hipStreamSynchronize(c->stream));
     HIP_TRY(hipMemcpy(out, src, F.ng * sizeof(double), hipMemcpyDeviceToHost));
+    return SQ_OK;
+}
+
+// Forces at the last SCF pass.  Local part on the host from phi (local_forces,
+// sq_setup.cpp).  Nonlocal part: the pass is repeated for the nodes that lie
+// in some projector box -- same V_eff, precision and early-exit settings, so
+// the same tridiagonals -- recording alpha of those boxes' images at every
+// Lanczos step; then c = f(T) e_1 per node and the contraction with grad chi at
+// the node (sq_forces.hip).  Nodes go in chunks sized to keep the record
+// under 1 GiB; per-node results are added per atom in node order.
+int sq_forces(sq_ctx *c, double *forces, double *f_local, double *f_nonlocal, sq_force_info *info)
+{
+    if (!c) return set_err(SQ_ERR_ARG, "null context");
+    if (!c->stream) return set_err(SQ_ERR_STATE, "host-only context (created with device < 0)");
+    auto &F = c->scf;
+    if (!F.force_ready) return set_err(SQ_ERR_STATE, "no SCF pass on this context (call sq_scf first)");
+    if (F.last_mode != SQ_NL_KB) return set_err(SQ_ERR_ARG, "forces need the Kleinman-Bylander operator (SQ_NL_KB)");
+    const System &S = c->sys;
+    const int nq = S.nqp;
+    if (nq < 1 || nq > kMaxNqp) return set_err(SQ_ERR_ARG, "forces need 1 <= N_qp <= 256");
+    HIP_TRY(hipSetDevice(c->device));
+    const auto t0 = std::chrono::steady_clock::now();
+    const int nsc = (int)S.sctype.size();
+    // ---- local part
+    std::vector<double> phi(F.ng), Floc;
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    HIP_TRY(hipMemcpy(phi.data(), F.phi, F.ng * sizeof(double), hipMemcpyDeviceToHost));
+    Status st = local_forces(S, phi.data(), F.b_scale, Floc);
+    if (!st.ok()) return set_err(st.code, st.msg);
+    // ---- nonlocal part: nodes inside a projector box, and the most boxes at a node
+    const auto t1 = std::chrono::steady_clock::now();
+    const long lx = S.box[3], ly = S.box[4], nn = S.n_nodes();
+    std::vector<int> cnt(nn, 0);
+    for (const Image &im : S.images) {
+        int lo[3], hi[3];
+        bool empty = false;
+        for (int d = 0; d < 3; d++) {
+            lo[d] = std::max(im.lo[d], S.box[d]);
+            hi[d] = std::min(im.hi[d], S.box[d] + S.box[3 + d] - 1);
+            empty = empty || lo[d] > hi[d];
+        }
+        if (empty) continue;
+        for (int k = lo[2]; k <= hi[2]; k++)
+            for (int j = lo[1]; j <= hi[1]; j++)
+                for (int i = lo[0]; i <= hi[0]; i++)
+                    cnt[((long)(k - S.box[2]) * ly + (j - S.box[1])) * lx + (i - S.box[0])]++;
+    }
+    std::vector<int> ids;
+    int slots = 0;
+    for (long q = 0; q < nn; q++)
+        if (cnt[q]) {
+            ids.push_back((int)q);
+            slots = std::max(slots, cnt[q]);
+        }
+    if (slots > kRecMax) return set_err(SQ_ERR_RANGE, "more than 32 projector boxes hold one node");
+    std::vector<double> Fnl((size_t)3 * std::max(nsc, S.nAtoms), 0.0);
+    const long nlist = (long)ids.size();
+    int chunks = 0;
+    if (nlist > 0) {
+        const int L = c->nlm_tmpl;
+        const size_t per_node = (size_t)nq * slots * L * sizeof(double) + (size_t)nq * sizeof(double) +
+                                (size_t)slots * (sizeof(int) + 3 * sizeof(double)) + sizeof(int);
+        const long chunk = std::min(nlist, std::max<long>(256, (long)((size_t)1 << 30) / (long)per_node));
+        std::vector<double> tab;
+        std::vector<TabType> types;
+        if (pack_projector_splines(S, tab, types)) return set_err(SQ_ERR_ARG, "more than 16 projector channels");
+        int *dids = nullptr, *drimg = nullptr;
+        double *drec = nullptr, *dcoef = nullptr, *dz = nullptr, *dout = nullptr, *dtab = nullptr;
+        TabType *dtyp = nullptr;
+        const long zg = std::min<long>(chunk, kFermiGrid);
+        struct Frees {
+            std::vector<void *> p;
+            ~Frees() { for (void *q : p) if (q) (void)hipFree(q); }
+        } frees;
+        auto alloc = [&](void **dst, size_t bytes) -> int {
+            HIP_TRY(hipMalloc(dst, std::max<size_t>(bytes, 8)));
+            frees.p.push_back(*dst);
+            return SQ_OK;
+        };
+        int rc;
+        if ((rc = alloc((void **)&dids, chunk * sizeof(int))) || (rc = alloc((void **)&drimg, chunk * slots * sizeof(int))) ||
+            (rc = alloc((void **)&drec, (size_t)chunk * nq * slots * L * sizeof(double))) ||
+            (rc = alloc((void **)&dcoef, (size_t)chunk * nq * sizeof(double))) ||
+            (rc = alloc((void **)&dz, (size_t)zg * nq * nq * sizeof(double))) ||
+            (rc = alloc((void **)&dout, (size_t)chunk * slots * 3 * sizeof(double))) ||
+            (rc = alloc((void **)&dtab, tab.size() * sizeof(double))) ||
+            (rc = alloc((void **)&dtyp, types.size() * sizeof(TabType))))
+            return rc;
+        HIP_TRY(hipMemcpy(dtab, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(dtyp, types.data(), types.size() * sizeof(TabType), hipMemcpyHostToDevice));
+        // the pass runs with the SCF pass's early-exit settings; the caller's come back after
+        struct ExitGuard {
+            sq_ctx *c;
+            double tol, lf, beta;
+            int mn, ev;
+            ~ExitGuard() { c->exit_tol = tol; c->exit_lf = lf; c->exit_beta = beta; c->exit_min = mn; c->exit_every = ev; }
+        } guard{c, c->exit_tol, c->exit_lf, c->exit_beta, c->exit_min, c->exit_every};
+        c->exit_tol = F.last_exit_tol;
+        c->exit_lf = F.last_exit_lf;
+        c->exit_beta = F.last_exit_beta;
+        c->exit_min = F.last_exit_min;
+        c->exit_every = F.last_exit_every;
+        const int prec = F.last_prec;
+        std::vector<double> hout;
+        std::vector<int> himg;
+        for (long b0 = 0; b0 < nlist; b0 += chunk) {
+            const long cn = std::min(chunk, nlist - b0);
+            HIP_TRY(hipMemcpyAsync(dids, ids.data() + b0, cn * sizeof(int), hipMemcpyHostToDevice, c->stream));
+            bool exit_used = false;
+            if (c->zmarch) {
+                KParams P = make_params(c, SQ_NL_KB, prec, 0, cn, true);
+                P.node_list = dids;
+                P.rec = drec;
+                P.rec_img = drimg;
+                P.rec_slots = slots;
+                exit_used = P.exit_tol > 0;
+                int grid = c->zgrid[prec];
+                if (cn < grid) grid = std::max<long>(8, (cn + 7) & ~7L);
+                HIP_TRY(hipMemsetAsync(c->queue, 0, 8 * sizeof(int), c->stream));
+                HIP_TRY(launch_zmarch(P, prec, grid, c->zlds, c->stream));
+            } else {
+                KParams P = make_params(c, SQ_NL_KB, prec, 0, cn);
+                P.node_list = dids;
+                P.rec = drec;
+                P.rec_img = drimg;
+                P.rec_slots = slots;
+                HIP_TRY(launch_lanczos(P, prec, (int)std::min<long>(c->grid, cn), c->lds, c->stream));
+            }
+            const int *ns = exit_used ? c->nsteps : nullptr;
+            HIP_TRY(launch_fermi_column(c->a, c->b, nq + 1, dids, ns, cn, nq, S.beta, F.lambda_f, dcoef, dz, c->status,
+                                        c->stream));
+            ForceParams fp;
+            memset(&fp, 0, sizeof fp);
+            fp.ids = dids;
+            fp.count = cn;
+            fp.slots = slots;
+            fp.nlm = L;
+            fp.nqp = nq;
+            fp.coef = dcoef;
+            fp.rec = drec;
+            fp.rec_img = drimg;
+            fp.nsteps = ns;
+            fp.img_cen = c->img_cen;
+            fp.img_type = c->img_type;
+            fp.types = dtyp;
+            fp.tab = dtab;
+            for (int d = 0; d < 3; d++) {
+                fp.h[d] = S.h[d];
+                fp.range[d] = S.range[d];
+                fp.box0[d] = S.box[d];
+                fp.box_n[d] = S.box[3 + d];
+            }
+            fp.out = dout;
+            fp.status = c->status;
+            HIP_TRY(launch_nl_force(fp, c->stream));
+            hout.resize((size_t)cn * slots * 3);
+            himg.resize((size_t)cn * slots);
+            HIP_TRY(hipMemcpyAsync(hout.data(), dout, hout.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+            HIP_TRY(hipMemcpyAsync(himg.data(), drimg, himg.size() * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+            HIP_TRY(hipStreamSynchronize(c->stream));
+            std::vector<int> stv(nn);
+            HIP_TRY(hipMemcpy(stv.data(), c->status, nn * sizeof(int), hipMemcpyDeviceToHost));
+            for (long r = 0; r < cn; r++) {
+                const int id = ids[b0 + r];
+                if (stv[id]) {
+                    char msg[160];
+                    snprintf(msg, sizeof msg, "force pass: node %d (rank-box id) status %d", id, stv[id]);
+                    const int code = stv[id] == 7 ? SQ_ERR_BREAKDOWN : stv[id] == 5 ? SQ_ERR_QL : stv[id] == 3 ? SQ_ERR_RANGE : SQ_ERR_ARG;
+                    return set_err(code, msg);
+                }
+                for (int sl = 0; sl < slots; sl++) {
+                    const int g = himg[(size_t)r * slots + sl];
+                    if (g < 0) continue;
+                    const int atom = S.images[g].atom;
+                    for (int d = 0; d < 3; d++) Fnl[3 * atom + d] += hout[((size_t)r * slots + sl) * 3 + d];
+                }
+            }
+            chunks++;
+        }
+    }
+    const auto t2 = std::chrono::steady_clock::now();
+    for (int q = 0; q < 3 * S.nAtoms; q++) {
+        if (forces) forces[q] = Floc[q] + Fnl[q];
+        if (f_local) f_local[q] = Floc[q];
+        if (f_nonlocal) f_nonlocal[q] = Fnl[q];
+    }
+    if (info) {
+        info->lambda_f = F.lambda_f;
+        info->wall_s = std::chrono::duration<double>(t2 - t0).count();
+        info->nonlocal_s = std::chrono::duration<double>(t2 - t1).count();
+        info->nodes = nlist;
+        info->slots = slots;
+        info->chunks = chunks;
+    }
     return SQ_OK;
 }
 

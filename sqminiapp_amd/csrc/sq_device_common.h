@@ -235,6 +235,47 @@ __device__ __forceinline__ T apply_point(const KParams &P, int p, int li, int lj
     return sum;
 }
 
+// ---- force pass records
+// The images of the node's list whose box holds the node itself (the only
+// ones whose projectors can be nonzero there), in list order, into near[];
+// their ids go to rec_img.  Returns the count (> rec_slots: the record
+// overflows and the node is flagged by the caller).
+__device__ __forceinline__ int record_near(const KParams &P, long r, const int *img_list, int nimg, const int node[3],
+                                           int *near, int *s_n)
+{
+    if (threadIdx.x == 0) {
+        int n = 0;
+        for (int li = 0; li < nimg; li++) {
+            const int g = img_list[li];
+            const int4 lo = P.img_lo[g], dm = P.img_dim[g];
+            if (node[0] >= lo.x && node[0] < lo.x + dm.x && node[1] >= lo.y && node[1] < lo.y + dm.y &&
+                node[2] >= lo.z && node[2] < lo.z + dm.z) {
+                if (n < P.rec_slots) near[n] = li;
+                n++;
+            }
+        }
+        *s_n = n;
+        for (int s = 0; s < P.rec_slots; s++) P.rec_img[r * P.rec_slots + s] = s < n ? img_list[near[s]] : -1;
+    }
+    __syncthreads();
+    return *s_n;
+}
+
+// alpha of the near images at Lanczos step `step` (q_step) into the record
+// (their count read from LDS, so no register carries it through the steps).
+__device__ __forceinline__ void record_step(const KParams &P, long r, int step, const int *near, const int &nnear_lds,
+                                            const double *alpha, int stride)
+{
+    if (step >= P.nqp) return;
+    const int nnear = nnear_lds;
+    const int L = P.nlm_max;
+    double *dst = P.rec + ((size_t)r * P.nqp + step) * P.rec_slots * L;
+    for (int e = threadIdx.x; e < nnear * L; e += blockDim.x) {
+        const int s = e / L, ch = e - s * L;
+        dst[s * L + ch] = alpha[near[s] * stride + ch];
+    }
+}
+
 // Node (K,J,I) of rank-box linear id (LIpos order).
 __device__ __forceinline__ void node_coords(const KParams &P, long id, int out[3])
 {

@@ -9,6 +9,7 @@ constexpr int kMaxChannels = 16;  // (l,m) channels per type for lmax <= 3
 constexpr int kMaxNqp = 256;      // Lanczos degree limit of the QL kernel
 constexpr int kMatsubara = 8;     // Matsubara points of the early-exit test
 constexpr int kMaxTypesDev = 8;
+constexpr int kRecMax = 32;       // image boxes holding one node, force pass
 
 // Everything a Lanczos / apply kernel reads; passed by value.
 struct KParams {
@@ -57,6 +58,12 @@ struct KParams {
     double exit_tol, exit_lf, exit_beta;
     int exit_min, exit_every;
     int *nsteps;                 // [node] Gauss nodes of the node's rule
+    // force pass: nodes from a list, and per Lanczos step the alpha of the
+    // images whose box holds the node (alpha = gamma <chi_atom, q_k>)
+    const int *node_list;        // [node_count] rank-box node ids (nullptr: node_begin + r)
+    double *rec;                 // [node_count][nqp][rec_slots][nlm_max]
+    int *rec_img;                // [node_count][rec_slots] image id, -1 past the node's count
+    int rec_slots;
 };
 
 hipError_t launch_lanczos(const KParams &P, int precision, int grid, size_t lds, hipStream_t st);
@@ -72,6 +79,33 @@ hipError_t launch_density_energy(const double *lam, const double *w, long n_node
                                  double two_over_dv, double *rho, double *part_e, double *part_s, int nblocks,
                                  double *out2, hipStream_t st);
 hipError_t launch_lam_range(const double *lmin, const double *lmax, long n, double *out, hipStream_t st);
+
+// ---- forces (sq_forces.hip)
+// coef[r][k] = (f(T) e_1)_k for node ids[r]'s tridiagonal (a, b at id * stride,
+// nsteps[id] rows when given), f the Fermi function; zws holds one n x n
+// eigenvector matrix per workgroup (min(count, kFermiGrid) workgroups).
+constexpr int kFermiGrid = 2048;
+hipError_t launch_fermi_column(const double *a, const double *b, int stride, const int *ids, const int *nsteps, long count,
+                               int nmax, double beta, double lf, double *coef, double *zws, int *status, hipStream_t st);
+struct TabType;
+struct ForceParams {
+    const int *ids;              // [count] node ids
+    long count;
+    int slots, nlm, nqp;         // rec layout [count][nqp][slots][nlm]
+    const double *coef;          // [count][nqp]
+    const double *rec;
+    const int *rec_img;
+    const int *nsteps;           // [node] or nullptr
+    const double4 *img_cen;
+    const int *img_type;
+    const TabType *types;
+    const double *tab;
+    double h[3], range[3];
+    int box0[3], box_n[3];
+    double *out;                 // [count][slots][3]
+    int *status;                 // [node]
+};
+hipError_t launch_nl_force(const ForceParams &F, hipStream_t st);
 
 // FP64 -> FP32 copy of a table (sq_setup.hip)
 hipError_t launch_to_float(const double *a, float *b, long n, hipStream_t st);

@@ -41,8 +41,9 @@ __global__ void __launch_bounds__(kWG) lanczos_kernel(KParams P)
     const T *diag_ext = reinterpret_cast<const T *>(P.diag_ext);
     const T *chi = reinterpret_cast<const T *>(P.chi);
 
+    __shared__ int s_near[kRecMax], s_nnear;
     for (long r = blockIdx.x; r < P.node_count; r += gridDim.x) {
-        const long id = P.node_begin + r;
+        const long id = P.node_list ? (long)P.node_list[r] : P.node_begin + r;
         int node[3];
         node_coords(P, id, node);
         int cube_lo[3] = {node[0] - P.hw[0], node[1] - P.hw[1], node[2] - P.hw[2]};
@@ -84,6 +85,13 @@ __global__ void __launch_bounds__(kWG) lanczos_kernel(KParams P)
             __syncthreads();
             continue;
         }
+        if (P.rec) {
+            if (record_near(P, r, img_list, nimg, node, s_near, &s_nnear) > P.rec_slots) {
+                if (threadIdx.x == 0) P.status[id] = 4;
+                __syncthreads();
+                continue;
+            }
+        }
         // Lanczos start: v = e_center, vprev = 0
         for (int p = threadIdx.x; p < nc; p += kWG) { buf0[p] = T(0); buf1[p] = T(0); }
         __syncthreads();
@@ -99,6 +107,7 @@ __global__ void __launch_bounds__(kWG) lanczos_kernel(KParams P)
             __syncthreads();
             images_alpha<NLM>(P, img_list, nimg, A, alpha);
             __syncthreads();
+            if (P.rec) record_step(P, r, step, s_near, s_nnear, alpha, NLM);
             double pa = 0.0;
             CSum ca{0.f, 0.f};
             for (int p = threadIdx.x; p < nc; p += kWG) {

@@ -10,6 +10,7 @@
 #include "sq_system.h"
 
 #include <algorithm>
+#include <array>
 #include <atomic>
 #include <cmath>
 #include <cstdio>
@@ -68,7 +69,7 @@ bool notaknot_slopes(const double *X, const double *Y, int n, std::vector<double
 // Cubic Hermite evaluation on the interval holding r (SplineEvaluate,
 // lsdft_tools.cc:15-68).  The interval is the one the reference's linear scan
 // stops at: the first knot >= r, stepped back unless r sits on it.
-bool RadialSpline::eval(double r, double *out) const
+bool RadialSpline::eval(double r, double *out, double *dout) const
 {
     if (!(r >= x[0] && r <= x[n - 1])) return false;
     int j = (int)(std::lower_bound(x, x + n, r) - x);
@@ -82,6 +83,7 @@ bool RadialSpline::eval(double r, double *out) const
     double A3 = dx * dx * (-2.0 * dy + yd[j] + yd[j + 1]);
     double t = r - p1;
     *out = ((A3 * t + A2) * t + A1) * t + A0;
+    if (dout) *dout = (3.0 * A3 * t + 2.0 * A2) * t + A1;
     return true;
 }
 
@@ -620,7 +622,7 @@ bool vloc_at(const System &S, int at, double r, double *out)
 }
 
 struct CellImage {
-    int at;
+    int at, atom;       // type, atom index
     double X[3];
     int lo[3], hi[3];   // box, unwrapped global indices
     bool self;          // image 0 of an atom
@@ -645,6 +647,7 @@ std::vector<CellImage> cell_images(const System &S, const int G[3])
                     for (int RR = -Imax[2]; RR <= Imax[2]; RR++) {
                         CellImage im;
                         im.at = at;
+                        im.atom = p;
                         im.X[0] = A[0] + PP * 2.0 * S.range[0];
                         im.X[1] = A[1] + QQ * 2.0 * S.range[1];
                         im.X[2] = A[2] + RR * 2.0 * S.range[2];
@@ -772,6 +775,97 @@ Status build_pseudocharge(const System &S, std::vector<double> &b, double &eself
     for (size_t q = 0; q < ng; q++) b[q] *= f;
     scale = f;
     eself = 0.5 * es * S.dV * f;
+    return {};
+}
+
+// Local forces.  With B = -(1/4pi) Lap_h (the pseudocharge stencil) and
+// g = grad V_loc sampled on an image's box widened by the stencil, the
+// derivative of b_J = B V_J with respect to the atom is -B g, so
+//   F_J = f dV sum_{J' of J} (B g_J')^T phi  -  (f dV / 2) [(B g_s)^T V_s + b_s^T g_s],
+// the first sum over the part of each image box inside the cell (where b is
+// placed), the self term over the atom's own whole box (b_s, V_s unscaled,
+// as E_self is formed).  Images are shared out to threads; each image's
+// partial sums are kept and added per atom in image order.
+Status local_forces(const System &S, const double *phi, double scale, std::vector<double> &F)
+{
+    const int G[3] = {S.npts[0] - 1, S.npts[1] - 1, S.npts[2] - 1};
+    const int o = S.order;
+    double sc[3][kMaxOrder + 1];
+    for (int d = 0; d < 3; d++)
+        for (int p = 0; p <= o; p++) sc[d][p] = S.coeff[d][p] / (2.0 * M_PI);
+    const std::vector<CellImage> imgs = cell_images(S, G);
+    std::vector<std::array<double, 6>> part(imgs.size());
+    std::atomic<size_t> next{0};
+    std::atomic<int> bad{0};
+    auto worker = [&]() {
+        std::vector<double> g, bj, vj;
+        for (size_t q; (q = next++) < imgs.size();) {
+            const CellImage &im = imgs[q];
+            int clo[3], chi[3], n[3], e[3];
+            bool empty = false;
+            for (int d = 0; d < 3; d++) {
+                clo[d] = std::max(im.lo[d], 0);
+                chi[d] = std::min(im.hi[d], G[d] - 1);
+                empty = empty || clo[d] > chi[d];
+                n[d] = im.hi[d] - im.lo[d] + 1;
+                e[d] = n[d] + 2 * o;
+            }
+            part[q].fill(0.0);
+            if (empty && !im.self) continue;
+            const Pseudopotential &P = S.psd[im.at];
+            g.assign((size_t)3 * e[0] * e[1] * e[2], 0.0);
+            for (int k = 0; k < e[2]; k++)
+                for (int j = 0; j < e[1]; j++)
+                    for (int i = 0; i < e[0]; i++) {
+                        const double x = S.h[0] * (im.lo[0] - o + i) - S.range[0] - im.X[0];
+                        const double y = S.h[1] * (im.lo[1] - o + j) - S.range[1] - im.X[1];
+                        const double z = S.h[2] * (im.lo[2] - o + k) - S.range[2] - im.X[2];
+                        const double r = std::sqrt(x * x + y * y + z * z);
+                        double v, dv;
+                        if (r > P.r[P.size - 1]) dv = (double)S.noe[im.at] / (r * r);
+                        else if (!P.s_vloc.eval(r, &v, &dv)) { bad = 3; return; }
+                        if (r < 1e-10) continue;
+                        double *gq = &g[3 * (((size_t)k * e[1] + j) * e[0] + i)];
+                        gq[0] = dv * x / r;
+                        gq[1] = dv * y / r;
+                        gq[2] = dv * z / r;
+                    }
+            if (im.self && !image_pseudocharge(S, im, im.lo, im.hi, sc, bj, &vj)) { bad = 3; return; }
+            auto GG = [&](int k, int j, int i, int d) { return g[3 * (((size_t)k * e[1] + j) * e[0] + i) + d]; };
+            double fphi[3] = {0, 0, 0}, fself[3] = {0, 0, 0};
+            for (int k = o; k < n[2] + o; k++)
+                for (int j = o; j < n[1] + o; j++)
+                    for (int i = o; i < n[0] + o; i++) {
+                        const int gi = im.lo[0] - o + i, gj = im.lo[1] - o + j, gk = im.lo[2] - o + k;
+                        const bool inside = !empty && gi >= clo[0] && gi <= chi[0] && gj >= clo[1] && gj <= chi[1] &&
+                                            gk >= clo[2] && gk <= chi[2];
+                        if (!inside && !im.self) continue;
+                        const size_t qb = ((size_t)(k - o) * n[1] + (j - o)) * n[0] + (i - o);
+                        for (int d = 0; d < 3; d++) {
+                            double v = GG(k, j, i, d) * (sc[0][0] + sc[1][0] + sc[2][0]);
+                            for (int a = 1; a <= o; a++)
+                                v += (GG(k, j, i - a, d) + GG(k, j, i + a, d)) * sc[0][a] +
+                                     (GG(k, j - a, i, d) + GG(k, j + a, i, d)) * sc[1][a] +
+                                     (GG(k - a, j, i, d) + GG(k + a, j, i, d)) * sc[2][a];
+                            if (inside) fphi[d] += v * phi[((size_t)gk * G[1] + gj) * G[0] + gi];
+                            if (im.self) fself[d] += v * vj[qb] + bj[qb] * GG(k, j, i, d);
+                        }
+                    }
+            for (int d = 0; d < 3; d++) { part[q][d] = fphi[d]; part[q][3 + d] = fself[d]; }
+        }
+    };
+    std::vector<std::thread> pool;
+    for (unsigned t = 0; t < setup_threads(); t++) pool.emplace_back(worker);
+    for (auto &t : pool) t.join();
+    if (bad) return fail(3, "local forces: V_loc spline evaluation failed");
+    std::vector<double> fphi((size_t)3 * S.nAtoms, 0.0), fself((size_t)3 * S.nAtoms, 0.0);
+    for (size_t q = 0; q < imgs.size(); q++)
+        for (int d = 0; d < 3; d++) {
+            fphi[3 * imgs[q].atom + d] += part[q][d];
+            fself[3 * imgs[q].atom + d] += part[q][3 + d];
+        }
+    F.assign((size_t)3 * S.nAtoms, 0.0);
+    for (size_t q = 0; q < F.size(); q++) F[q] = scale * S.dV * fphi[q] - 0.5 * scale * S.dV * fself[q];
     return {};
 }
 

@@ -4,12 +4,14 @@
 // one workgroup per image.  Every value is formed with the operations and
 // order of the host fill (sq_setup.cpp: RadialSpline::eval,
 // real_spherical_harmonic, fill_projectors) -- only +, -, *, / and sqrt, all
-// correctly rounded here -- so the tables are bit-identical to it.
+// correctly rounded here -- so the tables are bit-identical to it.  The spline
+// and harmonic functions are those of sq_projectors.h.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <vector>
 
+#include "sq_projectors.h"
 #include "sq_system.h"
 
 namespace sqamd {
@@ -24,72 +26,6 @@ struct TabImage {
     int type, nlm, nl;
     long long chi_off, rad_off;
 };
-
-struct TabType {
-    int r_off, n, lmax, lloc, nlm;
-    int y_off[4], yd_off[4];   // U_l dV_l values and not-a-knot slopes in the packed table
-    int ch_l[16], ch_m[16];
-};
-
-// SplineEvaluate on the interval holding r (first knot >= r, stepped back
-// unless r sits on it; the last knot takes the last interval).
-__device__ bool spline_at(const double *x, const double *y, const double *yd, int n, double r, double *out)
-{
-    if (!(r >= x[0] && r <= x[n - 1])) return false;
-    int lo = 0, hi = n;   // lower_bound
-    while (lo < hi) {
-        const int mid = (lo + hi) >> 1;
-        if (x[mid] < r) lo = mid + 1;
-        else hi = mid;
-    }
-    int j = lo;
-    if (r < x[j]) j--;
-    if (j > n - 2) j = n - 2;
-    const double p1 = x[j];
-    const double dx = 1.0 / (x[j + 1] - x[j]);
-    const double dy = (y[j + 1] - y[j]) * dx;
-    const double A0 = y[j], A1 = yd[j];
-    const double A2 = dx * (3.0 * dy - 2.0 * yd[j] - yd[j + 1]);
-    const double A3 = dx * dx * (-2.0 * dy + yd[j] + yd[j + 1]);
-    const double t = r - p1;
-    *out = ((A3 * t + A2) * t + A1) * t + A0;
-    return true;
-}
-
-// RealSphericalHarmonic for l <= 3 (lsdft_tools.cc:156-240), the reference's
-// 15-digit constants and expressions.
-__device__ double ylm(double x, double y, double z, int l, int m, bool *bad)
-{
-    const double c00 = 0.282094791773878;
-    const double c1 = 0.488602511902920;
-    const double c2a = 1.092548430592079, c20 = 0.315391565252520, c22 = 0.546274215296040;
-    const double c33 = 0.590043589926644, c32n = 2.890611442640554, c31 = 0.457045799464466;
-    const double c30 = 0.373176332590115, c32p = 1.445305721320277;
-    *bad = false;
-    const double r = sqrt(x * x + y * y + z * z);
-    if (l == 0) return c00;
-    if (l > 3 || m < -l || m > l) { *bad = true; return 0.0; }
-    if (!(r >= 1e-10)) return 0.0;
-    switch (l * 10 + (m + 3)) {
-    case 12: return c1 * (y / r);
-    case 13: return c1 * (z / r);
-    case 14: return c1 * (x / r);
-    case 21: return c2a * (x * y) / (r * r);
-    case 22: return c2a * (y * z) / (r * r);
-    case 23: return c20 * (-x * x - y * y + 2.0 * z * z) / (r * r);
-    case 24: return c2a * (z * x) / (r * r);
-    case 25: return c22 * (x * x - y * y) / (r * r);
-    case 30: return c33 * (3 * x * x - y * y) * y / (r * r * r);
-    case 31: return c32n * (x * y * z) / (r * r * r);
-    case 32: return c31 * y * (4 * z * z - x * x - y * y) / (r * r * r);
-    case 33: return c30 * z * (2 * z * z - 3 * x * x - 3 * y * y) / (r * r * r);
-    case 34: return c31 * x * (4 * z * z - x * x - y * y) / (r * r * r);
-    case 35: return c32p * z * (x * x - y * y) / (r * r * r);
-    case 36: return c33 * x * (x * x - 3 * y * y) / (r * r * r);
-    }
-    *bad = true;
-    return 0.0;
-}
 
 __global__ void __launch_bounds__(kTabThreads) projector_tables(const TabImage *__restrict__ imgs,
                                                                 const TabType *__restrict__ types,
@@ -154,12 +90,10 @@ hipError_t launch_to_float(const double *a, float *b, long n, hipStream_t st)
     return hipGetLastError();
 }
 
-int build_projector_tables(const System &S, double *chi, double *rad, int *status, void *stream)
+int pack_projector_splines(const System &S, std::vector<double> &tab, std::vector<TabType> &types)
 {
-    hipStream_t st = (hipStream_t)stream;
-    if (S.images.empty()) return 0;
-    std::vector<double> tab;
-    std::vector<TabType> types(S.ntype);
+    tab.clear();
+    types.assign(S.ntype, TabType{});
     for (int at = 0; at < S.ntype; at++) {
         const Pseudopotential &P = S.psd[at];
         TabType &T = types[at];
@@ -183,6 +117,16 @@ int build_projector_tables(const System &S, double *chi, double *rad, int *statu
             T.ch_m[c] = c < T.nlm ? P.ch_m[c] : 0;
         }
     }
+    return 0;
+}
+
+int build_projector_tables(const System &S, double *chi, double *rad, int *status, void *stream)
+{
+    hipStream_t st = (hipStream_t)stream;
+    if (S.images.empty()) return 0;
+    std::vector<double> tab;
+    std::vector<TabType> types;
+    if (int rc = pack_projector_splines(S, tab, types)) return rc;
     std::vector<TabImage> imgs(S.images.size());
     for (size_t g = 0; g < S.images.size(); g++) {
         const Image &src = S.images[g];

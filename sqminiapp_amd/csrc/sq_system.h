@@ -23,8 +23,9 @@ struct RadialSpline {
     const double *x = nullptr;  // abscissae (owned by the Pseudopotential)
     std::vector<double> y, yd;
     int n = 0;
-    // Evaluates at r; returns false when r is outside [x0, x_{n-1}].
-    bool eval(double r, double *out) const;
+    // Evaluates at r (and, with dout, the cubic's derivative); returns false
+    // when r is outside [x0, x_{n-1}].
+    bool eval(double r, double *out, double *dout = nullptr) const;
 };
 
 struct Pseudopotential {
@@ -120,6 +121,10 @@ Status build_system(const std::string &dir, const std::string &name, const Setup
 // densities scaled to N_e.
 Status build_pseudocharge(const System &S, std::vector<double> &b, double &eself, double &scale);
 Status build_atomic_density(const System &S, std::vector<double> &rho, double &scale);
+// Pseudocharge forces F[3 * nAtoms] for the potential phi on the periodic cell
+// (the derivative of (1/2) int phi (b - rho) - E_self at fixed rho; scale is
+// the neutrality factor build_pseudocharge returned).
+Status local_forces(const System &S, const double *phi, double scale, std::vector<double> &F);
 
 // chi and rad of every image on the device (sq_setup.hip), with the host
 // fill's arithmetic: the tables come out bit-identical.  chi/rad: device

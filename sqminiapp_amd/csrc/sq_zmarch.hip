@@ -127,6 +127,7 @@ __global__ void __launch_bounds__(WG) lanczos_zmarch(KParams P)
     __shared__ double s_ab[2][kMaxNqp + 1];
     __shared__ double s_g[kMatsubara][2];
     __shared__ int s_flag;
+    __shared__ int s_near[kRecMax], s_nnear;
     for (;;) {
         if (tid == 0) {
             long r = -1;
@@ -143,7 +144,7 @@ __global__ void __launch_bounds__(WG) lanczos_zmarch(KParams P)
         const long r = s_next;
         __syncthreads();
         if (r < 0) break;
-        const long id = P.node_begin + r;
+        const long id = P.node_list ? (long)P.node_list[r] : P.node_begin + r;
         int node[3];
         node_coords(P, id, node);
         const int cube_lo[3] = {node[0] - P.hw[0], node[1] - P.hw[1], node[2] - P.hw[2]};
@@ -191,6 +192,13 @@ __global__ void __launch_bounds__(WG) lanczos_zmarch(KParams P)
             if (tid == 0) P.status[id] = 4;
             __syncthreads();
             continue;
+        }
+        if (P.rec) {
+            if (record_near(P, r, img_list, nimg, node, s_near, &s_nnear) > P.rec_slots) {
+                if (tid == 0) P.status[id] = 4;
+                __syncthreads();
+                continue;
+            }
         }
         // ---- per column: bitmask of the node's images whose x/y box holds it
         unsigned cmask[CPT][MW];
@@ -332,6 +340,7 @@ __global__ void __launch_bounds__(WG) lanczos_zmarch(KParams P)
             __syncthreads();
             images_alpha<NLM>(P, img_list, nimg, A, alpha);
             __syncthreads();
+            if (P.rec) record_step(P, r, step, s_near, s_nnear, alpha, NLM);
             // ---- nonlocal part of <v, H v> from the gathered sums: the scatter adds
             // sum_ch chi_ch alpha_ch per image, so <v, V_nl v> = sum alpha . A
             double pn = 0.0;
