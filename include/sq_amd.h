@@ -256,7 +256,9 @@ int sq_poisson(sq_ctx *ctx, const double *f, double tol, double *phi, int *itera
  *              node's Gauss-quadrature density-matrix column Q f(T) e_1, from a
  *              repeat of the pass over the nodes inside projector boxes.
  * forces, f_local, f_nonlocal: 3 * nAtoms each (x, y, z per atom, .ion order);
- * any may be NULL.  Needs the Kleinman-Bylander operator (SQ_NL_KB). */
+ * any may be NULL, and a part is computed only when forces or its own array
+ * is given (f_local alone skips the nonlocal pass).  Needs the
+ * Kleinman-Bylander operator (SQ_NL_KB). */
 typedef struct {
     double lambda_f;         /* Fermi level of the pass */
     double wall_s;           /* whole call */

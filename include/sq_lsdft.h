@@ -56,9 +56,15 @@ struct SQ_LSDFT {
     std::vector<double> elecDensRho;
     double lambda_f = 0;
 
+    /* forces (ilsdft.h:163-164,299-300): x, y, z per atom in .ion order, and
+     * their local / nonlocal parts */
+    std::vector<double> forces, localforces, nonlocalforces;
+    double MaxForce = 0, NormForce = 0;   /* largest component; mean |F_J| */
+
     int rank = 0;
     std::string DenFilename;     /* "./<name>" (lsdft_readfiles.cc:118,136) */
     std::string ReadDenFilename;
+    std::string ForceFilename;   /* "./<name>.force" (lsdft_readfiles.cc:115,124-125) */
 };
 
 /* Get_Input + Initialize + GetInfluencingAtoms_Local/_Nonlocal (main.cc:31-38;
@@ -116,6 +122,17 @@ int SystemEnergy_Calc_lssgq(SQ_LSDFT *pLsdft);
  * Eband, Entropy, Exc, Eself, Etot; returns 0 or the sq_status code
  * (SQ_ERR_RANGE when the loop ends unconverged). */
 double SCF_LSSGQ(SQ_LSDFT *pLsdft);
+
+/* Calculate_localforce / Calculate_nonlocalforce (declared, lsdft.h:188-189):
+ * the pseudocharge and Kleinman-Bylander parts of the forces at the last pass
+ * of SCF_LSSGQ (sq_forces) into localforces / nonlocalforces; forces is their
+ * sum once both have run, with MaxForce and NormForce. */
+int Calculate_localforce(SQ_LSDFT *pLsdft);
+int Calculate_nonlocalforce(SQ_LSDFT *pLsdft);
+
+/* Display_force (declared, lsdft.h:191): prints the forces and writes them to
+ * ForceFilename, one line "Fx Fy Fz" (Ha/bohr) per atom. */
+int Display_force(SQ_LSDFT *pLsdft);
 
 /* The reference's per-rank text files (lsdft_readfiles.cc:1106-1380):
  * <DenFilename>_<rank>.den holds "xcor ycor zcor lx ly lz" then "k j i rho"

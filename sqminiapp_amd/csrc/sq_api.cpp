@@ -1254,17 +1254,22 @@ int sq_forces(sq_ctx *c, double *forces, double *f_local, double *f_nonlocal, sq
     HIP_TRY(hipSetDevice(c->device));
     const auto t0 = std::chrono::steady_clock::now();
     const int nsc = (int)S.sctype.size();
+    const bool want_local = forces || f_local, want_nl = forces || f_nonlocal;
     // ---- local part
-    std::vector<double> phi(F.ng), Floc;
-    HIP_TRY(hipStreamSynchronize(c->stream));
-    HIP_TRY(hipMemcpy(phi.data(), F.phi, F.ng * sizeof(double), hipMemcpyDeviceToHost));
-    Status st = local_forces(S, phi.data(), F.b_scale, Floc);
-    if (!st.ok()) return set_err(st.code, st.msg);
+    std::vector<double> Floc((size_t)3 * S.nAtoms, 0.0);
+    if (want_local) {
+        std::vector<double> phi(F.ng);
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        HIP_TRY(hipMemcpy(phi.data(), F.phi, F.ng * sizeof(double), hipMemcpyDeviceToHost));
+        Status st = local_forces(S, phi.data(), F.b_scale, Floc);
+        if (!st.ok()) return set_err(st.code, st.msg);
+    }
     // ---- nonlocal part: nodes inside a projector box, and the most boxes at a node
     const auto t1 = std::chrono::steady_clock::now();
     const long lx = S.box[3], ly = S.box[4], nn = S.n_nodes();
-    std::vector<int> cnt(nn, 0);
+    std::vector<int> cnt(want_nl ? nn : 0, 0);
     for (const Image &im : S.images) {
+        if (!want_nl) break;
         int lo[3], hi[3];
         bool empty = false;
         for (int d = 0; d < 3; d++) {
@@ -1280,7 +1285,7 @@ int sq_forces(sq_ctx *c, double *forces, double *f_local, double *f_nonlocal, sq
     }
     std::vector<int> ids;
     int slots = 0;
-    for (long q = 0; q < nn; q++)
+    for (long q = 0; q < nn && want_nl; q++)
         if (cnt[q]) {
             ids.push_back((int)q);
             slots = std::max(slots, cnt[q]);

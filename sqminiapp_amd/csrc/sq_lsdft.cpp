@@ -3,6 +3,7 @@
 // SQ_LSDFT arrays; no arithmetic of the path runs here.
 #include "../../include/sq_lsdft.h"
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -78,6 +79,10 @@ int Initialize(SQ_LSDFT *p, const char *dir, const char *name, const sq_options 
     p->lambda_max.assign(n, 0.0);
     p->elecDensRho.assign(n, 0.0);
     if (p->DenFilename.empty()) p->DenFilename = std::string("./") + name;
+    if (p->ForceFilename.empty()) p->ForceFilename = std::string("./") + name + ".force";
+    p->forces.assign(3 * (size_t)p->nAtoms, 0.0);
+    p->localforces.assign(3 * (size_t)p->nAtoms, 0.0);
+    p->nonlocalforces.assign(3 * (size_t)p->nAtoms, 0.0);
     return SQ_OK;
 }
 
@@ -192,6 +197,55 @@ double SCF_LSSGQ(SQ_LSDFT *p)
                                        p->lambda_max.data())))
         return rc;
     return r.converged ? SQ_OK : fail(SQ_ERR_RANGE, "SCF_LSSGQ: not converged in MAXIT_SCF iterations");
+}
+
+// forces = local + nonlocal, MaxForce = the largest |component|, NormForce =
+// the mean over atoms of |F_J|
+static void sum_forces(SQ_LSDFT *p)
+{
+    double mx = 0.0, nrm = 0.0;
+    for (int a = 0; a < p->nAtoms; a++) {
+        double s2 = 0.0;
+        for (int d = 0; d < 3; d++) {
+            const double f = p->localforces[3 * a + d] + p->nonlocalforces[3 * a + d];
+            p->forces[3 * a + d] = f;
+            mx = std::max(mx, std::fabs(f));
+            s2 += f * f;
+        }
+        nrm += std::sqrt(s2);
+    }
+    p->MaxForce = mx;
+    p->NormForce = p->nAtoms > 0 ? nrm / p->nAtoms : 0.0;
+}
+
+int Calculate_localforce(SQ_LSDFT *p)
+{
+    if (!ready(p)) return fail(SQ_ERR_STATE, "Calculate_localforce: Initialize first");
+    int rc = sq_forces(p->ctx, nullptr, p->localforces.data(), nullptr, nullptr);
+    if (!rc) sum_forces(p);
+    return rc;
+}
+
+int Calculate_nonlocalforce(SQ_LSDFT *p)
+{
+    if (!ready(p)) return fail(SQ_ERR_STATE, "Calculate_nonlocalforce: Initialize first");
+    int rc = sq_forces(p->ctx, nullptr, nullptr, p->nonlocalforces.data(), nullptr);
+    if (!rc) sum_forces(p);
+    return rc;
+}
+
+int Display_force(SQ_LSDFT *p)
+{
+    if (!ready(p)) return fail(SQ_ERR_STATE, "Display_force: Initialize first");
+    printf("Atomic forces (Ha/Bohr):\n");
+    for (int a = 0; a < p->nAtoms; a++)
+        printf("%4d  % .10E  % .10E  % .10E\n", a, p->forces[3 * a], p->forces[3 * a + 1], p->forces[3 * a + 2]);
+    printf("Maximum force component: %.10E   mean |F|: %.10E\n", p->MaxForce, p->NormForce);
+    FILE *fp = fopen(p->ForceFilename.c_str(), "w");
+    if (!fp) return fail(SQ_ERR_IO, "Couldn't create output file for Forces " + p->ForceFilename);
+    for (int a = 0; a < p->nAtoms; a++)
+        fprintf(fp, "%.16E %.16E %.16E\n", p->forces[3 * a], p->forces[3 * a + 1], p->forces[3 * a + 2]);
+    return fclose(fp) == 0 ? SQ_OK : fail(SQ_ERR_IO, "write failed: " + p->ForceFilename);
 }
 
 int Print_ElecDens(SQ_LSDFT *p)

@@ -3,14 +3,16 @@
 //
 //   sq_main -name <dir>/<name> [-mode collapsed|kb] [-fp32] [-phantom [x y z]]
 //           [-rank_box xcor ycor zcor lx ly lz] [-rank r] [-nqp n] [-device d]
-//           [-out <DenFilename>] [-density] [-verify_files] [-scf]
+//           [-out <DenFilename>] [-density] [-verify_files] [-scf [-forces]]
 //
 // It prints the reference's progress lines and its two SQ timers, then writes
 // <DenFilename>_<rank>.lambdas (Print_MinMaxLambda) and, with -density, the
 // Fermi level, energies and <DenFilename>_<rank>.den (Print_ElecDens).  Any
 // failing step prints sq_last_error() and exits with status 1, where the
 // reference exits too.  -scf runs SCF_LSSGQ on the periodic cell instead of
-// one SQ pass (the node box becomes the cell) and prints its energies.
+// one SQ pass (the node box becomes the cell) and prints its energies; with
+// -forces (Kleinman-Bylander mode, -mode kb) it then runs Calculate_localforce,
+// Calculate_nonlocalforce and Display_force, which writes ./<name>.force.
 // -verify_files reads the written files back with
 // Read_MinMaxLambda / Read_ElecDens and checks them against the results.
 #include <algorithm>
@@ -41,7 +43,7 @@ int main(int argc, char **argv)
     SQ_LSDFT lsdft;
     sq_options opt;
     sq_default_options(&opt);
-    bool density = false, verify = false, scf = false;
+    bool density = false, verify = false, scf = false, forces = false;
     std::string out;
     for (int a = 1; a < argc; a++) {
         const std::string s = argv[a];
@@ -72,9 +74,11 @@ int main(int argc, char **argv)
         else if (s == "-density") density = true;
         else if (s == "-verify_files") verify = true;
         else if (s == "-scf") scf = true;
+        else if (s == "-forces") forces = true;
         else { fprintf(stderr, "unknown option %s\n", s.c_str()); return 1; }
     }
     if (path.empty()) { fprintf(stderr, "usage: sq_main -name <dir>/<name> [options]\n"); return 1; }
+    if (forces && !scf) { fprintf(stderr, "-forces needs -scf\n"); return 1; }
     const size_t slash = path.find_last_of('/');
     const std::string dir = slash == std::string::npos ? "." : path.substr(0, slash);
     const std::string name = slash == std::string::npos ? path : path.substr(slash + 1);
@@ -116,6 +120,13 @@ int main(int argc, char **argv)
         SystemEnergy_Calc_lssgq(&lsdft);
         density = false;   // rho of the SCF is already in elecDensRho
         if ((rc = Print_ElecDens(&lsdft))) die("Print_ElecDens", rc);
+        if (forces) {
+            const double f1 = now();
+            if ((rc = Calculate_localforce(&lsdft))) die("Calculate_localforce", rc);
+            if ((rc = Calculate_nonlocalforce(&lsdft))) die("Calculate_nonlocalforce", rc);
+            printf("rank=%d, Time spent in forces=%lf \n", lsdft.rank, now() - f1);
+            if ((rc = Display_force(&lsdft))) die("Display_force", rc);
+        }
     } else if ((rc = (int)NodesAndWeights_LSSGQ(&lsdft))) die("NodesAndWeights_LSSGQ", rc);
     const double v2 = now();
     const clock_t end = clock();

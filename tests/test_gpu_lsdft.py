@@ -54,17 +54,22 @@ def test_driver_mg1_kb_against_golden(tmp_path):
 
 
 def test_driver_scf_matches_the_api(tmp_path):
-    """sq_main -scf: SCF_LSSGQ on the Mg1 cell with the deck's SCF keywords
-    (MIXING_PARAMETER 0.05, history 7, Pulay every 2nd step, TOL_SCF 1e-5)."""
-    print("sq_main Mg1 -scf", flush=True)
-    r = subprocess.run([EXE, "-name", os.path.join(MG1, "Mg1"), "-mode", "kb", "-scf", "-out", str(tmp_path / "Mg1")],
-                       capture_output=True, text=True, cwd=str(tmp_path), timeout=300)
+    """sq_main -scf -forces: SCF_LSSGQ on the Mg1 cell with the deck's SCF
+    keywords (MIXING_PARAMETER 0.05, history 7, Pulay every 2nd step, TOL_SCF
+    1e-5), then Calculate_localforce / Calculate_nonlocalforce / Display_force."""
+    print("sq_main Mg1 -scf -forces", flush=True)
+    r = subprocess.run([EXE, "-name", os.path.join(MG1, "Mg1"), "-mode", "kb", "-scf", "-forces", "-out",
+                        str(tmp_path / "Mg1")], capture_output=True, text=True, cwd=str(tmp_path), timeout=300)
     assert r.returncode == 0, r.stderr
     e = float(re.search(r"Total energy\s+=\s+(\S+)", r.stdout).group(1))
     its = int(re.search(r"SCF converged in (\d+) iterations", r.stdout).group(1))
     with sq.Context(MG1, "Mg1", cell=True) as c:
         res = c.scf(sq.SQ_NL_KB, sq.SQ_FP64)
+        F, _, _, _ = c.forces()
     assert res["converged"] and res["iterations"] == its
+    Ff = np.loadtxt(tmp_path / "Mg1.force")
+    assert Ff.shape == (4, 3) and np.allclose(Ff, F, rtol=0, atol=1e-15)
+    assert "Maximum force component" in r.stdout
     assert e == pytest.approx(res["e_total"], abs=1e-11)
     den = np.loadtxt(tmp_path / "Mg1_0.den", skiprows=1)
     assert den.shape[0] == 10 * 17 * 16

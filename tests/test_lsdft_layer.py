@@ -24,7 +24,8 @@ def test_reference_named_entry_points_are_exported():
     for must in ("NodesAndWeights_LSSGQ", "CalculateSpectralNodesAndWeights", "Mult_NonlocalVector",
                  "Mult_HamiltonianVector", "NonlocalProjectors", "CalculateNonlocalProjectors",
                  "TridiagEigenVecSolve_NodesAndWeights", "CalculateDensity_lssgq", "Print_ElecDens",
-                 "Read_ElecDens", "Print_MinMaxLambda", "Read_MinMaxLambda"):
+                 "Read_ElecDens", "Print_MinMaxLambda", "Read_MinMaxLambda", "Calculate_localforce",
+                 "Calculate_nonlocalforce", "Display_force"):
         assert must in names
     out = subprocess.run(["nm", "-D", "-C", "--defined-only", LIB], capture_output=True, text=True, check=True).stdout
     exported = set(re.findall(r"\s[TW]\s+([A-Za-z_]+)\(SQ_LSDFT", out))
@@ -42,6 +43,11 @@ def test_driver_usage_and_bad_options(tmp_path):
     assert r.returncode == 1 and "usage" in r.stderr
     r = _run(["-name", os.path.join(MG1, "Mg1"), "-mode", "bogus"], str(tmp_path))
     assert r.returncode == 1
+
+
+def test_driver_forces_need_scf(tmp_path):
+    r = _run(["-name", os.path.join(MG1, "Mg1"), "-forces", "-device", "-1"], str(tmp_path))
+    assert r.returncode == 1 and "-forces needs -scf" in r.stderr
 
 
 def test_driver_missing_deck_exits_1(tmp_path):
