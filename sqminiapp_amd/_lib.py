@@ -26,7 +26,9 @@ EXPORTS = ["sq_last_error", "sq_default_options", "sq_create_from_deck", "sq_des
            "sq_run", "sq_get_nodes_and_weights", "sq_get_lanczos", "sq_nodes_and_weights", "sq_last_timing",
            "sq_apply_h", "sq_apply_nonlocal", "sq_tridiag_nodes_weights", "sq_density_fermi_energy", "sq_synchronize",
            "sq_get_image", "sq_default_scf_options", "sq_scf", "sq_get_cell_field", "sq_set_veff_cell", "sq_xc",
-           "sq_poisson", "sq_set_early_exit", "sq_get_rule_sizes", "sq_forces"]
+           "sq_poisson", "sq_set_early_exit", "sq_get_rule_sizes", "sq_forces", "sq_mlp_create", "sq_mlp_destroy",
+           "sq_mlp_get_params", "sq_mlp_set_params", "sq_mlp_descriptors", "sq_mlp_add_sample", "sq_mlp_train",
+           "sq_mlp_forward", "sq_mlp_loss_grad", "sq_mlp_guess"]
 
 FIELDS = {"rho": 0, "rho_out": 1, "phi": 2, "vxc": 3, "veff": 4, "b": 5, "rho_atomic": 6}
 
@@ -116,6 +118,17 @@ def lib():
     L.sq_set_early_exit.argtypes = [vp, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int]
     L.sq_get_rule_sizes.argtypes = [vp, ip]
     L.sq_forces.argtypes = [vp, dp, dp, dp, C.POINTER(ForceInfo)]
+    fp = C.POINTER(C.c_float)
+    L.sq_mlp_create.argtypes = [C.c_int, C.c_uint, C.POINTER(vp)]
+    L.sq_mlp_destroy.argtypes = [vp]
+    L.sq_mlp_get_params.argtypes = [vp, fp, fp]
+    L.sq_mlp_set_params.argtypes = [vp, fp, fp]
+    L.sq_mlp_descriptors.argtypes = [vp, fp]
+    L.sq_mlp_add_sample.argtypes = [vp, vp]
+    L.sq_mlp_train.argtypes = [vp, C.c_int, C.c_double, dp]
+    L.sq_mlp_forward.argtypes = [vp, fp, lg, C.c_int, C.c_int, fp]
+    L.sq_mlp_loss_grad.argtypes = [vp, fp, fp, lg, dp, dp]
+    L.sq_mlp_guess.argtypes = [vp, vp, C.c_int, dp]
     L.sq_tridiag_nodes_weights.argtypes = [vp, dp, dp, C.c_int, lg, dp, dp, dp, dp]
     L.sq_density_fermi_energy.argtypes = [vp, C.c_double, C.c_double, dp, dp, dp, dp]
     L.sq_synchronize.argtypes = [vp]
@@ -292,6 +305,12 @@ class Context:
         info = {k: getattr(fi, k) for k, _ in ForceInfo._fields_}
         return F.reshape(-1, 3), Fl.reshape(-1, 3), Fn.reshape(-1, 3), info
 
+    def descriptors(self):
+        """Initial-density network descriptors of the cell, [points, 16] (i fastest)."""
+        out = np.zeros((int(np.prod(self.cell_shape())), Mlp.K), np.float32)
+        _check(lib().sq_mlp_descriptors(self._h, _fp(out)))
+        return out
+
     def cell_field(self, name):
         out = np.zeros(int(np.prod(self.cell_shape())))
         _check(lib().sq_get_cell_field(self._h, FIELDS[name], _dp(out)))
@@ -349,3 +368,73 @@ class Context:
             chi = np.zeros(ent.value)
             _check(lib().sq_get_image(self._h, int(c), None, None, None, None, None, _dp(chi)))
         return dict(center=cen, box=box, atom=atom.value, type=typ.value, entries=ent.value, chi=chi)
+
+
+def _fp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float)) if a is not None else None
+
+
+class Mlp:
+    """The initial-density network (sq_mlp_*): descriptors -> correction to
+    the superposed atomic density, trained on converged SQ densities."""
+    K, H, PARAMS, NORM = 16, 64, 5313, 33
+
+    def __init__(self, device=0, seed=0):
+        self._h = C.c_void_p()
+        _check(lib().sq_mlp_create(int(device), int(seed), C.byref(self._h)))
+
+    def close(self):
+        if self._h:
+            lib().sq_mlp_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def params(self):
+        p = np.zeros(self.PARAMS, np.float32)
+        n = np.zeros(self.NORM, np.float32)
+        _check(lib().sq_mlp_get_params(self._h, _fp(p), _fp(n)))
+        return p, n
+
+    def set_params(self, p, norm=None):
+        p = np.ascontiguousarray(p, np.float32)
+        n = None if norm is None else np.ascontiguousarray(norm, np.float32)
+        _check(lib().sq_mlp_set_params(self._h, _fp(p), _fp(n)))
+
+    def add_sample(self, ctx):
+        _check(lib().sq_mlp_add_sample(self._h, ctx._h))
+
+    def train(self, epochs, lr=3e-3):
+        loss = C.c_double()
+        _check(lib().sq_mlp_train(self._h, int(epochs), float(lr), C.byref(loss)))
+        return loss.value
+
+    def forward(self, x, bf16=False, normalize=True):
+        x = np.ascontiguousarray(x, np.float32)
+        y = np.zeros(x.shape[0], np.float32)
+        _check(lib().sq_mlp_forward(self._h, _fp(x), x.shape[0], int(bool(bf16)), int(bool(normalize)), _fp(y)))
+        return y
+
+    def loss_grad(self, x, t):
+        x = np.ascontiguousarray(x, np.float32)
+        t = np.ascontiguousarray(t, np.float32)
+        loss = C.c_double()
+        g = np.zeros(self.PARAMS)
+        _check(lib().sq_mlp_loss_grad(self._h, _fp(x), _fp(t), x.shape[0], C.byref(loss), _dp(g)))
+        return loss.value, g
+
+    def guess(self, ctx, bf16=False):
+        """Initial density for ctx's cell (cell shape), for Context.scf(rho0=...)."""
+        out = np.zeros(int(np.prod(ctx.cell_shape())))
+        _check(lib().sq_mlp_guess(self._h, ctx._h, int(bool(bf16)), _dp(out)))
+        return out.reshape(ctx.cell_shape())

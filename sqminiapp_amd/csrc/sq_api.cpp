@@ -1433,6 +1433,337 @@ int sq_forces(sq_ctx *c, double *forces, double *f_local, double *f_nonlocal, sq
     return SQ_OK;
 }
 
+}  // extern "C"
+
+// ------------------------------------------------------------ initial-density network
+// The model and its training set live beside the contexts: descriptors and
+// targets (rho - rho_atomic) of converged SCFs are gathered on the host, then
+// trained on the device in full batches (sq_mlp.hip).
+
+struct sq_mlp {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int K = kMlpK, H = kMlpH, np = 0;
+    double rf = 6.0;
+    float *p = nullptr, *m = nullptr, *v = nullptr;
+    double *g = nullptr;
+    int t = 0;
+    bool have_norm = false;
+    std::vector<float> mean, rstd;
+    double tscale = 1.0;
+    std::vector<float> X, T;
+};
+
+namespace {
+
+struct DevBufs {
+    std::vector<void *> p;
+    ~DevBufs() { for (void *q : p) if (q) (void)hipFree(q); }
+    template <typename T>
+    int get(T **dst, size_t n)
+    {
+        HIP_TRY(hipMalloc((void **)dst, std::max<size_t>(n, 1) * sizeof(T)));
+        p.push_back(*dst);
+        return SQ_OK;
+    }
+};
+
+// every image (shifts 2R) of the atoms that can lie within rf of the cell,
+// the positions first wrapped into [-R, R)
+std::vector<double4> mlp_images(const System &S, double rf)
+{
+    int n[3];
+    for (int d = 0; d < 3; d++) n[d] = (int)std::ceil(rf / (2.0 * S.range[d])) + 1;
+    std::vector<double4> out;
+    for (int a = 0; a < S.nAtoms; a++) {
+        double A[3];
+        for (int d = 0; d < 3; d++) {
+            const double L = 2.0 * S.range[d];
+            A[d] = S.atompos[3 * a + d] - L * std::floor((S.atompos[3 * a + d] + S.range[d]) / L);
+        }
+        for (int i = -n[0]; i <= n[0]; i++)
+            for (int j = -n[1]; j <= n[1]; j++)
+                for (int k = -n[2]; k <= n[2]; k++)
+                    out.push_back(make_double4(A[0] + 2.0 * S.range[0] * i, A[1] + 2.0 * S.range[1] * j,
+                                               A[2] + 2.0 * S.range[2] * k, 0.0));
+    }
+    return out;
+}
+
+// descriptors of the context's cell into a device buffer (ng * K floats)
+int mlp_descriptors_dev(sq_ctx *c, double rf, float *dout)
+{
+    const System &S = c->sys;
+    int G[3], rc;
+    if ((rc = cell_dims(c, G))) return rc;
+    std::vector<double4> img = mlp_images(S, rf);
+    DevBufs B;
+    double4 *dimg;
+    if ((rc = B.get(&dimg, img.size()))) return rc;
+    HIP_TRY(hipMemcpy(dimg, img.data(), img.size() * sizeof(double4), hipMemcpyHostToDevice));
+    const double h[3] = {S.h[0], S.h[1], S.h[2]}, R[3] = {S.range[0], S.range[1], S.range[2]};
+    HIP_TRY(launch_mlp_features(dimg, (int)img.size(), G, h, R, kMlpK, rf, dout, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return SQ_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int sq_mlp_create(int device, unsigned seed, sq_mlp **out)
+{
+    if (!out) return set_err(SQ_ERR_ARG, "null argument");
+    *out = nullptr;
+    auto *m = new sq_mlp();
+    m->device = device;
+    m->np = mlp_param_count(m->K, m->H);
+    m->mean.assign(m->K, 0.0f);
+    m->rstd.assign(m->K, 1.0f);
+    // uniform(-a, a) with a = sqrt(6 / (fan_in + fan_out)); biases zero (xorshift generator)
+    std::vector<float> p(m->np, 0.0f);
+    uint64_t st = 0x9E3779B97F4A7C15ull ^ seed;
+    auto uni = [&]() {
+        st ^= st << 13; st ^= st >> 7; st ^= st << 17;
+        return (double)(st >> 11) / 9007199254740992.0 * 2.0 - 1.0;
+    };
+    const int K = m->K, H = m->H;
+    auto fill = [&](int off, int n, int fi, int fo) {
+        const double a = std::sqrt(6.0 / (fi + fo));
+        for (int q = 0; q < n; q++) p[off + q] = (float)(a * uni());
+    };
+    fill(0, H * K, K, H);
+    fill(H * K + H, H * H, H, H);
+    fill(H * K + H + H * H + H, H, H, 1);
+    int rc;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&m->stream) != hipSuccess) rc = set_err(SQ_ERR_DEVICE, "no device");
+    else if (hipMalloc((void **)&m->p, m->np * sizeof(float)) != hipSuccess ||
+             hipMalloc((void **)&m->m, m->np * sizeof(float)) != hipSuccess ||
+             hipMalloc((void **)&m->v, m->np * sizeof(float)) != hipSuccess ||
+             hipMalloc((void **)&m->g, m->np * sizeof(double)) != hipSuccess)
+        rc = set_err(SQ_ERR_ALLOC, "network buffers");
+    else if (hipMemcpy(m->p, p.data(), m->np * sizeof(float), hipMemcpyHostToDevice) != hipSuccess ||
+             hipMemset(m->m, 0, m->np * sizeof(float)) != hipSuccess || hipMemset(m->v, 0, m->np * sizeof(float)) != hipSuccess)
+        rc = set_err(SQ_ERR_DEVICE, "network upload");
+    else rc = SQ_OK;
+    if (rc) { sq_mlp_destroy(m); return rc; }
+    *out = m;
+    return SQ_OK;
+}
+
+int sq_mlp_destroy(sq_mlp *m)
+{
+    if (!m) return SQ_OK;
+    (void)hipSetDevice(m->device);
+    for (void *q : {(void *)m->p, (void *)m->m, (void *)m->v, (void *)m->g})
+        if (q) (void)hipFree(q);
+    if (m->stream) (void)hipStreamDestroy(m->stream);
+    delete m;
+    return SQ_OK;
+}
+
+int sq_mlp_get_params(sq_mlp *m, float *params, float *norm)
+{
+    if (!m) return set_err(SQ_ERR_ARG, "null model");
+    HIP_TRY(hipSetDevice(m->device));
+    if (params) HIP_TRY(hipMemcpy(params, m->p, m->np * sizeof(float), hipMemcpyDeviceToHost));
+    if (norm) {
+        for (int q = 0; q < m->K; q++) { norm[q] = m->mean[q]; norm[m->K + q] = m->rstd[q]; }
+        norm[2 * m->K] = (float)m->tscale;
+    }
+    return SQ_OK;
+}
+
+int sq_mlp_set_params(sq_mlp *m, const float *params, const float *norm)
+{
+    if (!m) return set_err(SQ_ERR_ARG, "null model");
+    HIP_TRY(hipSetDevice(m->device));
+    if (params) HIP_TRY(hipMemcpy(m->p, params, m->np * sizeof(float), hipMemcpyHostToDevice));
+    if (norm) {
+        for (int q = 0; q < m->K; q++) { m->mean[q] = norm[q]; m->rstd[q] = norm[m->K + q]; }
+        m->tscale = norm[2 * m->K];
+        m->have_norm = true;
+    }
+    HIP_TRY(hipMemset(m->m, 0, m->np * sizeof(float)));
+    HIP_TRY(hipMemset(m->v, 0, m->np * sizeof(float)));
+    m->t = 0;
+    return SQ_OK;
+}
+
+int sq_mlp_descriptors(sq_ctx *c, float *out)
+{
+    if (!c || !out) return set_err(SQ_ERR_ARG, "null argument");
+    if (!c->stream) return set_err(SQ_ERR_STATE, "host-only context (created with device < 0)");
+    int G[3], rc;
+    if ((rc = cell_dims(c, G))) return rc;
+    HIP_TRY(hipSetDevice(c->device));
+    const long ng = (long)G[0] * G[1] * G[2];
+    DevBufs B;
+    float *d;
+    if ((rc = B.get(&d, ng * kMlpK)) || (rc = mlp_descriptors_dev(c, 6.0, d))) return rc;
+    HIP_TRY(hipMemcpy(out, d, ng * kMlpK * sizeof(float), hipMemcpyDeviceToHost));
+    return SQ_OK;
+}
+
+int sq_mlp_add_sample(sq_mlp *m, sq_ctx *c)
+{
+    if (!m || !c) return set_err(SQ_ERR_ARG, "null argument");
+    if (!c->stream) return set_err(SQ_ERR_STATE, "host-only context (created with device < 0)");
+    if (!c->scf.force_ready) return set_err(SQ_ERR_STATE, "no SCF density on this context (call sq_scf first)");
+    const System &S = c->sys;
+    int G[3], rc;
+    if ((rc = cell_dims(c, G))) return rc;
+    const long ng = (long)G[0] * G[1] * G[2];
+    std::vector<float> f(ng * kMlpK);
+    if ((rc = sq_mlp_descriptors(c, f.data()))) return rc;
+    std::vector<double> rho(ng), rat;
+    double sc;
+    if ((rc = sq_get_cell_field(c, SQ_FIELD_RHO, rho.data()))) return rc;
+    Status st = build_atomic_density(S, rat, sc);
+    if (!st.ok()) return set_err(st.code, st.msg);
+    m->X.insert(m->X.end(), f.begin(), f.end());
+    for (long q = 0; q < ng; q++) m->T.push_back((float)(rho[q] - rat[q]));
+    return SQ_OK;
+}
+
+// L and dL/dp (double) of the current parameters on normalized data already on
+// the device (X: N x K, T: N), with the activations and deltas in scratch
+static int mlp_loss_grad_dev(sq_mlp *m, const float *X, const float *T, long N, double *loss, DevBufs &B)
+{
+    const int K = m->K, H = m->H;
+    float *H1, *H2, *Y, *D1, *D2, *D3;
+    double *part, *lsum;
+    const int nchunk = (int)std::max<long>(1, std::min<long>(128, N / 256));
+    int rc;
+    if ((rc = B.get(&H1, N * H)) || (rc = B.get(&H2, N * H)) || (rc = B.get(&Y, N)) || (rc = B.get(&D1, N * H)) ||
+        (rc = B.get(&D2, N * H)) || (rc = B.get(&D3, N)) || (rc = B.get(&part, (size_t)nchunk * H * H)) ||
+        (rc = B.get(&lsum, 1)))
+        return rc;
+    HIP_TRY(launch_mlp_forward(X, N, m->p, nullptr, nullptr, H1, H2, Y, 0, m->stream));
+    HIP_TRY(launch_mlp_deltas(H1, H2, Y, T, N, m->p, D1, D2, D3, m->stream));
+    double *g = m->g;
+    HIP_TRY(launch_outer_sum(D1, H, X, K, N, part, nchunk, g, m->stream));                          // W1
+    HIP_TRY(launch_outer_sum(D1, H, nullptr, 1, N, part, nchunk, g + H * K, m->stream));            // b1
+    HIP_TRY(launch_outer_sum(D2, H, H1, H, N, part, nchunk, g + H * K + H, m->stream));             // W2
+    HIP_TRY(launch_outer_sum(D2, H, nullptr, 1, N, part, nchunk, g + H * K + H + H * H, m->stream)); // b2
+    HIP_TRY(launch_outer_sum(H2, H, D3, 1, N, part, nchunk, g + H * K + 2 * H + H * H, m->stream));  // w3
+    HIP_TRY(launch_outer_sum(D3, 1, nullptr, 1, N, part, nchunk, g + H * K + 3 * H + H * H, m->stream)); // b3
+    if (loss) {
+        HIP_TRY(launch_outer_sum(D3, 1, D3, 1, N, part, nchunk, lsum, m->stream));
+        double s2;
+        HIP_TRY(hipMemcpyAsync(&s2, lsum, sizeof(double), hipMemcpyDeviceToHost, m->stream));
+        HIP_TRY(hipStreamSynchronize(m->stream));
+        *loss = 0.5 * (double)N * s2;   // d3 = e / N
+    }
+    return SQ_OK;
+}
+
+int sq_mlp_loss_grad(sq_mlp *m, const float *X, const float *T, long N, double *loss, double *grad)
+{
+    if (!m || !X || !T || N <= 0) return set_err(SQ_ERR_ARG, "bad argument");
+    HIP_TRY(hipSetDevice(m->device));
+    DevBufs B;
+    float *dX, *dT;
+    int rc;
+    if ((rc = B.get(&dX, N * m->K)) || (rc = B.get(&dT, N))) return rc;
+    HIP_TRY(hipMemcpy(dX, X, N * m->K * sizeof(float), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(dT, T, N * sizeof(float), hipMemcpyHostToDevice));
+    if ((rc = mlp_loss_grad_dev(m, dX, dT, N, loss, B))) return rc;
+    if (grad) HIP_TRY(hipMemcpy(grad, m->g, m->np * sizeof(double), hipMemcpyDeviceToHost));
+    HIP_TRY(hipStreamSynchronize(m->stream));
+    return SQ_OK;
+}
+
+int sq_mlp_forward(sq_mlp *m, const float *X, long N, int bf16, int normalize, float *Y)
+{
+    if (!m || !X || !Y || N <= 0) return set_err(SQ_ERR_ARG, "bad argument");
+    HIP_TRY(hipSetDevice(m->device));
+    DevBufs B;
+    float *dX, *dY, *dmean, *drstd;
+    int rc;
+    if ((rc = B.get(&dX, N * m->K)) || (rc = B.get(&dY, N)) || (rc = B.get(&dmean, m->K)) || (rc = B.get(&drstd, m->K)))
+        return rc;
+    HIP_TRY(hipMemcpy(dX, X, N * m->K * sizeof(float), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(dmean, m->mean.data(), m->K * sizeof(float), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(drstd, m->rstd.data(), m->K * sizeof(float), hipMemcpyHostToDevice));
+    HIP_TRY(launch_mlp_forward(dX, N, m->p, normalize ? dmean : nullptr, normalize ? drstd : nullptr, nullptr, nullptr, dY,
+                               bf16, m->stream));
+    HIP_TRY(hipMemcpyAsync(Y, dY, N * sizeof(float), hipMemcpyDeviceToHost, m->stream));
+    HIP_TRY(hipStreamSynchronize(m->stream));
+    return SQ_OK;
+}
+
+int sq_mlp_train(sq_mlp *m, int epochs, double lr, double *loss)
+{
+    if (!m || epochs < 0) return set_err(SQ_ERR_ARG, "bad argument");
+    const long N = (long)m->T.size();
+    if (N == 0) return set_err(SQ_ERR_STATE, "no training samples (sq_mlp_add_sample first)");
+    HIP_TRY(hipSetDevice(m->device));
+    const int K = m->K;
+    if (!m->have_norm) {   // descriptor standardization and target scale from the first training set
+        for (int q = 0; q < K; q++) {
+            double s = 0, s2 = 0;
+            for (long n = 0; n < N; n++) { const double x = m->X[n * K + q]; s += x; s2 += x * x; }
+            const double mu = s / N, var = std::max(0.0, s2 / N - mu * mu);
+            m->mean[q] = (float)mu;
+            m->rstd[q] = (float)(1.0 / std::max(std::sqrt(var), 1e-6));
+        }
+        double s2 = 0;
+        for (long n = 0; n < N; n++) s2 += (double)m->T[n] * m->T[n];
+        m->tscale = std::max(std::sqrt(s2 / N), 1e-12);
+        m->have_norm = true;
+    }
+    std::vector<float> Xn(N * K), Tn(N);
+    for (long n = 0; n < N; n++) {
+        for (int q = 0; q < K; q++) Xn[n * K + q] = (m->X[n * K + q] - m->mean[q]) * m->rstd[q];
+        Tn[n] = (float)(m->T[n] / m->tscale);
+    }
+    DevBufs B;
+    float *dX, *dT;
+    int rc;
+    if ((rc = B.get(&dX, N * K)) || (rc = B.get(&dT, N))) return rc;
+    HIP_TRY(hipMemcpy(dX, Xn.data(), N * K * sizeof(float), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(dT, Tn.data(), N * sizeof(float), hipMemcpyHostToDevice));
+    for (int e = 0; e < epochs; e++) {
+        DevBufs S;
+        if ((rc = mlp_loss_grad_dev(m, dX, dT, N, nullptr, S))) return rc;
+        HIP_TRY(launch_adam(m->p, m->g, m->m, m->v, m->np, (float)lr, ++m->t, m->stream));
+        if ((e & 63) == 63) HIP_TRY(hipStreamSynchronize(m->stream));
+    }
+    if (loss) {
+        DevBufs S;
+        if ((rc = mlp_loss_grad_dev(m, dX, dT, N, loss, S))) return rc;
+    }
+    HIP_TRY(hipStreamSynchronize(m->stream));
+    return SQ_OK;
+}
+
+int sq_mlp_guess(sq_mlp *m, sq_ctx *c, int bf16, double *rho0)
+{
+    if (!m || !c || !rho0) return set_err(SQ_ERR_ARG, "null argument");
+    if (!c->stream) return set_err(SQ_ERR_STATE, "host-only context (created with device < 0)");
+    const System &S = c->sys;
+    int G[3], rc;
+    if ((rc = cell_dims(c, G))) return rc;
+    const long ng = (long)G[0] * G[1] * G[2];
+    std::vector<float> f(ng * kMlpK), y(ng);
+    if ((rc = sq_mlp_descriptors(c, f.data())) || (rc = sq_mlp_forward(m, f.data(), ng, bf16, 1, y.data()))) return rc;
+    std::vector<double> rat;
+    double sc;
+    Status st = build_atomic_density(S, rat, sc);
+    if (!st.ok()) return set_err(st.code, st.msg);
+    double tot = 0.0;
+    for (long q = 0; q < ng; q++) {
+        rho0[q] = std::max(0.0, rat[q] + m->tscale * (double)y[q]);
+        tot += rho0[q];
+    }
+    tot *= S.dV;
+    if (!(tot > 0)) return set_err(SQ_ERR_RANGE, "network density has no charge");
+    const double fsc = (double)S.noetot / tot;
+    for (long q = 0; q < ng; q++) rho0[q] *= fsc;
+    return SQ_OK;
+}
+
 int sq_set_veff_cell(sq_ctx *c, const double *veff)
 {
     if (!c || !veff) return set_err(SQ_ERR_ARG, "null argument");

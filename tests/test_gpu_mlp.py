@@ -1,0 +1,105 @@
+"""The initial-density network on the GPU (sq_mlp_*) against the numpy
+restatement (oracle/mlp.py), and what it gains over the superposition of
+atomic densities.
+
+- descriptors of a displaced Mg1 cell against oracle.mlp.features (FP32 storage);
+- forward: FP32 against numpy float64, and the BF16-storage form against
+  numpy with the same roundings;
+- loss and gradient against numpy (FP32 activations, double sums);
+- training on converged SQ densities of displaced Mg1 cells lowers the loss,
+  and on a held-out cell the learned guess starts closer to the converged
+  density and the SCF needs fewer iterations than from the superposition,
+  ending at the same energy.
+"""
+import numpy as np
+import pytest
+
+from conftest import MG1, write_variant
+
+import sqminiapp_amd as sq
+
+pytestmark = pytest.mark.gpu
+K, H = 16, 64
+
+
+def _cell(tmp, name, seed, sigma=0.15):
+    from oracle import oracle as O
+    pos = O.System(MG1, "Mg1").atom_positions() + np.random.default_rng(seed).normal(0.0, sigma, (4, 3))
+    write_variant(str(tmp), name, atoms=pos)
+    return str(tmp), name, pos
+
+
+def _bf16_forward(p, x):
+    from oracle import mlp as M
+    W1, b1, W2, b2, w3, b3 = [np.asarray(M.bf16(np.asarray(a, np.float32)), np.float64) if np.ndim(a) else
+                              float(M.bf16(np.float32([a]))[0]) for a in M.unpack(p, K, H)]
+    xh = M.bf16(x.astype(np.float32)).astype(np.float64)
+    h1 = M.bf16(np.tanh(xh @ W1.T + b1).astype(np.float32)).astype(np.float64)
+    h2 = M.bf16(np.tanh(h1 @ W2.T + b2).astype(np.float32)).astype(np.float64)
+    return h2 @ w3 + b3
+
+
+def test_descriptors_match_numpy(tmp_path):
+    from oracle import mlp as M
+    d, n, pos = _cell(tmp_path, "D", 1)
+    with sq.Context(d, n, cell=True) as c:
+        f = c.descriptors()
+        I = c.info
+        G = (I.npts[0] - 1, I.npts[1] - 1, I.npts[2] - 1)
+        ref = M.features(pos, list(I.delta), list(I.range), G)
+    assert f.shape == ref.shape
+    assert np.max(np.abs(f - ref)) < 1e-6 * max(1.0, np.max(np.abs(ref)))
+    assert np.max(ref) > 0.5
+
+
+def test_forward_and_gradient_match_numpy():
+    from oracle import mlp as M
+    rng = np.random.default_rng(7)
+    x = rng.normal(0, 1, (3000, K)).astype(np.float32)
+    t = rng.normal(0, 1, 3000).astype(np.float32)
+    with sq.Mlp(seed=5) as m:
+        p, _ = m.params()
+        y = m.forward(x, normalize=False)
+        y_ref = M.forward(p.astype(np.float64), x.astype(np.float64), K, H)[0]
+        assert np.max(np.abs(y - y_ref)) < 2e-5 * max(1.0, np.max(np.abs(y_ref)))
+        yb = m.forward(x, bf16=True, normalize=False)
+        yb_ref = _bf16_forward(p.astype(np.float64), x)
+        assert np.max(np.abs(yb - yb_ref)) < 2e-3 * max(1.0, np.max(np.abs(yb_ref)))
+        assert np.max(np.abs(yb - y)) < 5e-2 * max(1.0, np.max(np.abs(y)))   # bf16 keeps 8 bits
+        L, g = m.loss_grad(x, t)
+        L_ref, g_ref = M.loss_grad(p.astype(np.float64), x.astype(np.float64), t.astype(np.float64), K, H)
+        assert L == pytest.approx(L_ref, rel=1e-5)
+        assert np.max(np.abs(g - g_ref)) < 1e-4 * np.max(np.abs(g_ref))
+
+
+def test_learned_guess_beats_the_superposition(tmp_path):
+    kw = dict(precision=sq.SQ_FP64, tol=1e-7, max_iter=200, mixing=0.3)
+    with sq.Mlp(seed=1) as m:
+        for s in range(2, 8):
+            d, n, _ = _cell(tmp_path / f"t{s}", "D", s)
+            with sq.Context(d, n, cell=True) as c:
+                r = c.scf(**kw)
+                assert r["converged"]
+                m.add_sample(c)
+        loss0 = m.train(0)
+        loss = m.train(2000, lr=3e-3)
+        print(f"training loss {loss0:.4f} -> {loss:.4f} (scaled units)", flush=True)
+        assert loss < 0.2 * loss0
+        d, n, _ = _cell(tmp_path / "held", "D", 99)
+        with sq.Context(d, n, cell=True) as c:
+            ra = c.scf(**kw)
+            rho = c.cell_field("rho")
+            atomic = c.cell_field("rho_atomic")
+            atomic = atomic * (c.info.noetot / (atomic.sum() * c.info.dV))
+            guess = m.guess(c)
+            gb = m.guess(c, bf16=True)
+            rl = c.scf(rho0=guess, **kw)
+        e_at = np.linalg.norm(atomic - rho) / np.linalg.norm(rho)
+        e_ml = np.linalg.norm(guess - rho) / np.linalg.norm(rho)
+        e_bf = np.linalg.norm(gb - rho) / np.linalg.norm(rho)
+        print(f"held-out cell: initial error {e_at:.4f} (atoms) -> {e_ml:.4f} (network FP32), {e_bf:.4f} (BF16); "
+              f"SCF {ra['iterations']} -> {rl['iterations']} iterations", flush=True)
+        assert guess.sum() * c.info.dV == pytest.approx(c.info.noetot, rel=1e-12)
+        assert e_ml < 0.6 * e_at and e_bf < 0.7 * e_at
+        assert rl["converged"] and rl["iterations"] < ra["iterations"]
+        assert abs(rl["e_total"] - ra["e_total"]) / c.info.nAtoms < 1e-6
