@@ -57,12 +57,28 @@ def scf_wall_time(d, name, max_iter, mode):
     against FP64 (the metric's 1e-5 Ha/atom); then the forces of each form
     (the metric's 1e-4 Ha/bohr), their time reported beside the SCF's."""
     import sqminiapp_amd as sq
+    from sqminiapp_amd import train_density_net
     with sq.Context(d, name, device=0, cell=True) as c:
         I = c.info
         r64 = c.scf(mode, sq.SQ_FP64, tol=1e-6, max_iter=max_iter)
         F64, _, _, fi64 = c.forces() if mode == sq.SQ_NL_KB else (None, None, None, None)
         r32 = c.scf(mode, sq.SQ_FP32, tol=1e-5, max_iter=max_iter)
         F32, _, _, fi32 = c.forces() if mode == sq.SQ_NL_KB else (None, None, None, None)
+        # the chosen form: FP32 from the learned initial density (weights trained on other cells)
+        net = train_density_net.load() if mode == sq.SQ_NL_KB else None
+        rl = Fl = None
+        if net is not None:
+            t0 = time.perf_counter()
+            guess = net.guess(c, bf16=True)
+            t_guess = time.perf_counter() - t0
+            rl = c.scf(mode, sq.SQ_FP32, tol=1e-5, max_iter=max_iter, rho0=guess)
+            Fl, _, _, fil = c.forces()
+            atomic = c.cell_field("rho_atomic")
+            atomic = atomic * (I.noetot / (atomic.sum() * I.dV))
+            rho = c.cell_field("rho")
+            err = (float(np.linalg.norm(atomic - rho) / np.linalg.norm(rho)),
+                   float(np.linalg.norm(guess - rho) / np.linalg.norm(rho)))
+            net.close()
     keep = ("iterations", "converged", "residual", "wall_s", "sq_s", "poisson_s", "e_total", "lambda_f")
     f64 = {k: r64[k] for k in keep}
     f32 = {k: r32[k] for k in keep}
@@ -75,6 +91,14 @@ def scf_wall_time(d, name, max_iter, mode):
         out["forces"] = {"fp64_s": fi64["wall_s"], "fp32_s": fi32["wall_s"], "nodes": fi64["nodes"],
                          "max_abs_fp64": float(abs(F64).max()), "max_diff_fp32_fp64": float(abs(F32 - F64).max()),
                          "unit": "Ha/bohr"}
+    if rl is not None:
+        wall = rl["wall_s"] + t_guess
+        out["chosen_form"] = {"form": "FP32 Lanczos from the learned initial density (BF16 network inference)",
+                              **{k: rl[k] for k in keep}, "guess_s": t_guess, "wall_with_guess_s": wall,
+                              "initial_error_atoms": err[0], "initial_error_network": err[1],
+                              "d_energy_per_atom": abs(rl["e_total"] - r64["e_total"]) / I.nAtoms,
+                              "max_force_diff_vs_fp64": float(abs(Fl - F64).max()),
+                              "wall_speedup_vs_fp64": r64["wall_s"] / wall}
     return out
 
 
