@@ -51,7 +51,7 @@ def cpu_baseline(deck_dir, name, n_sample, threads):
 KERNEL_NAMES = {0: "lanczos_kernel (generic)", 1: "lanczos_zmarch (column-marching)"}
 
 
-def scf_wall_time(d, name, max_iter, mode):
+def scf_wall_time(d, name, max_iter, mode, verbose=False):
     """SCF wall time on the periodic cell of the same deck: plain FP64 to a
     density residual of 1e-6, and the FP32 form to 1e-5, its energy judged
     against FP64 (the metric's 1e-5 Ha/atom); then the forces of each form
@@ -60,9 +60,9 @@ def scf_wall_time(d, name, max_iter, mode):
     from sqminiapp_amd import train_density_net
     with sq.Context(d, name, device=0, cell=True) as c:
         I = c.info
-        r64 = c.scf(mode, sq.SQ_FP64, tol=1e-6, max_iter=max_iter)
+        r64 = c.scf(mode, sq.SQ_FP64, tol=1e-6, max_iter=max_iter, verbose=verbose)
         F64, _, _, fi64 = c.forces() if mode == sq.SQ_NL_KB else (None, None, None, None)
-        r32 = c.scf(mode, sq.SQ_FP32, tol=1e-5, max_iter=max_iter)
+        r32 = c.scf(mode, sq.SQ_FP32, tol=1e-5, max_iter=max_iter, verbose=verbose)
         F32, _, _, fi32 = c.forces() if mode == sq.SQ_NL_KB else (None, None, None, None)
         # the chosen form: FP32 from the learned initial density (weights trained on other cells)
         net = train_density_net.load() if mode == sq.SQ_NL_KB else None
@@ -71,7 +71,7 @@ def scf_wall_time(d, name, max_iter, mode):
             t0 = time.perf_counter()
             guess = net.guess(c, bf16=True)
             t_guess = time.perf_counter() - t0
-            rl = c.scf(mode, sq.SQ_FP32, tol=1e-5, max_iter=max_iter, rho0=guess)
+            rl = c.scf(mode, sq.SQ_FP32, tol=1e-5, max_iter=max_iter, rho0=guess, verbose=verbose)
             Fl, _, _, fil = c.forces()
             atomic = c.cell_field("rho_atomic")
             atomic = atomic * (I.noetot / (atomic.sum() * I.dV))
@@ -122,6 +122,8 @@ def main():
     ap.add_argument("--fp32-steps", type=int, default=None)
     ap.add_argument("--no-scf", action="store_true", help="skip the SCF wall-time measurement")
     ap.add_argument("--scf-iter", type=int, default=60)
+    ap.add_argument("--scf-config", default=None, help="deck of the SCF measurement (default: --config)")
+    ap.add_argument("--scf-verbose", action="store_true", help="one line per SCF iteration on stderr")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: the SQ node loop would shard across GPUs by 3-D node blocks "
@@ -185,7 +187,11 @@ def main():
                             "lanczos_ms": round(float(k32[:, 0].mean()), 3),
                             "achieved_GBs": round(b32 / (float(k32[:, 0].mean()) * 1e-3) / 1e9, 1)}
     if not args.no_scf:
-        out["scf"] = scf_wall_time(d, name, args.scf_iter, mode)
+        if args.scf_config and args.scf_config != args.config:
+            ds, ns = decks.deck(args.scf_config, root)
+        else:
+            ds, ns = d, name
+        out["scf"] = scf_wall_time(ds, ns, args.scf_iter, mode, args.scf_verbose)
     if not args.no_cpu_baseline:
         threads = max(1, min(16, os.cpu_count() or 1))
         out["cpu_baseline"] = cpu_baseline(d, name, args.cpu_sample, threads)

@@ -197,7 +197,7 @@ typedef struct {
     int pulay_frequency;     /* <= 0: PULAY_FREQUENCY */
     double tol_poisson;      /* <= 0: TOL_POISSON */
     const double *rho0;      /* initial density on the cell; NULL: superposed atomic densities */
-    int verbose;             /* 1: one line per iteration on stdout */
+    int verbose;             /* 1: one line per iteration on stderr */
     double exit_tol;         /* > 0: per-node early exit (sq_set_early_exit) from the second
                                 iteration on, at the previous iteration's Fermi level */
     int exit_min, exit_every;   /* its first check and spacing (<= 0: 30 and 5) */

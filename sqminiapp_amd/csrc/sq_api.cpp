@@ -96,6 +96,9 @@ struct sq_ctx {
         bool have_b = false;
         double eself = 0, b_scale = 0;
         std::vector<double> host_b;
+        // four CG steps of the Poisson solve, captured once (the buffers above are fixed)
+        hipGraph_t cg_graph = nullptr;
+        hipGraphExec_t cg_exec = nullptr;
         // the last SQ pass of the loop, which sq_forces repeats: its input
         // density's V_eff (left in the diagonal), phi, Fermi level and settings
         bool force_ready = false;
@@ -126,6 +129,8 @@ void free_ctx(sq_ctx *c)
     void *scf_ptrs[] = {c->scf.rho, c->scf.rho_out, c->scf.b, c->scf.phi, c->scf.eps, c->scf.vxc, c->scf.veff, c->scf.r,
                         c->scf.p, c->scf.Ap, c->scf.f, c->scf.x_prev, c->scf.f_prev, c->scf.dX, c->scf.dF, c->scf.g,
                         c->scf.s, c->scf.part};
+    if (c->scf.cg_exec) (void)hipGraphExecDestroy(c->scf.cg_exec);
+    if (c->scf.cg_graph) (void)hipGraphDestroy(c->scf.cg_graph);
     for (void *p : scf_ptrs)
         if (p) (void)hipFree(p);
     for (void *p : ptrs)
@@ -928,6 +933,11 @@ static int scf_alloc(sq_ctx *c, int hist)
     auto &F = c->scf;
     const long ng = c->sys.n_nodes();
     if (F.rho && F.ng == ng && F.hist >= hist) return SQ_OK;
+    // the captured CG block names the old buffers
+    if (F.cg_exec) (void)hipGraphExecDestroy(F.cg_exec);
+    if (F.cg_graph) (void)hipGraphDestroy(F.cg_graph);
+    F.cg_exec = nullptr;
+    F.cg_graph = nullptr;
     double **all[] = {&F.rho, &F.rho_out, &F.b, &F.phi, &F.eps, &F.vxc, &F.veff, &F.r, &F.p, &F.Ap, &F.f, &F.x_prev,
                       &F.f_prev, &F.dX, &F.dF, &F.g, &F.s, &F.part};
     for (double **q : all) {
@@ -1004,13 +1014,38 @@ static int scf_poisson(sq_ctx *c, const double *f, double tol, int max_it, int *
     double rr = 0.0;
     HIP_TRY(hipMemcpyAsync(&rr, F.s, sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
+    // One CG step is seven launches (stencil, two two-stage dots, two updates),
+    // all on fixed buffers with their scalars on the device: four steps are
+    // captured once as a graph and replayed between the host's convergence
+    // checks (SQ_NO_GRAPHS set: launched one by one, the same kernels in the
+    // same order).
+    auto cg_step = [&]() -> int {
+        HIP_TRY(launch_lap_periodic(F.p, F.Ap, G, S.order, sc, c->stream));
+        if (int r2 = scf_dot(c, F.p, F.Ap, 1, nullptr)) return r2;
+        HIP_TRY(launch_cg_xr(F.phi, F.r, F.p, F.Ap, F.s, n, c->stream));
+        if (int r2 = scf_dot(c, F.r, F.r, 2, nullptr)) return r2;
+        HIP_TRY(launch_cg_p(F.p, F.r, F.s, n, c->stream));
+        return SQ_OK;
+    };
+    const bool graphs = getenv("SQ_NO_GRAPHS") == nullptr;
     while (rr > stop && it < max_it) {
-        for (int k = 0; k < 4 && it < max_it; k++, it++) {
-            HIP_TRY(launch_lap_periodic(F.p, F.Ap, G, S.order, sc, c->stream));
-            if ((rc = scf_dot(c, F.p, F.Ap, 1, nullptr))) return rc;
-            HIP_TRY(launch_cg_xr(F.phi, F.r, F.p, F.Ap, F.s, n, c->stream));
-            if ((rc = scf_dot(c, F.r, F.r, 2, nullptr))) return rc;
-            HIP_TRY(launch_cg_p(F.p, F.r, F.s, n, c->stream));
+        if (graphs && it + 4 <= max_it) {
+            if (!F.cg_exec) {
+                HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+                int rc2 = SQ_OK;
+                for (int k = 0; k < 4 && !rc2; k++) rc2 = cg_step();
+                hipGraph_t g = nullptr;
+                const hipError_t e = hipStreamEndCapture(c->stream, &g);
+                if (rc2) { if (g) (void)hipGraphDestroy(g); return rc2; }
+                if (e != hipSuccess) return set_err(SQ_ERR_DEVICE, "Poisson: graph capture failed");
+                F.cg_graph = g;
+                HIP_TRY(hipGraphInstantiate(&F.cg_exec, g, nullptr, nullptr, 0));
+            }
+            HIP_TRY(hipGraphLaunch(F.cg_exec, c->stream));
+            it += 4;
+        } else {
+            for (int k = 0; k < 4 && it < max_it; k++, it++)
+                if ((rc = cg_step())) return rc;
         }
         HIP_TRY(hipMemcpyAsync(&rr, F.s, sizeof(double), hipMemcpyDeviceToHost, c->stream));
         HIP_TRY(hipStreamSynchronize(c->stream));
@@ -1160,7 +1195,7 @@ int sq_scf(sq_ctx *c, const sq_scf_options *opt_in, sq_scf_result *res)
         res->e_total = eb + res->e_xc - res->e_vxc + res->e_el - F.eself + mts;
         res->poisson_iterations = pits;
         if (opt.verbose)
-            printf("scf %3d  residual %.3e  E %.12f  lambda_f %.10f  poisson %d\n", it, res->residual, res->e_total, lf, pits);
+            fprintf(stderr, "scf %3d  residual %.3e  E %.12f  lambda_f %.10f  poisson %d\n", it, res->residual, res->e_total, lf, pits);
         if (res->residual < tol) { res->converged = 1; break; }
         // Periodic Pulay: history of iterate and residual differences
         if (it > 1) {

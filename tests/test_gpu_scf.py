@@ -99,3 +99,26 @@ def test_scf_mg1_fp64_and_fp32_forms():
     assert r64["converged"] and r32["converged"]
     assert de < 1e-5
     assert abs(r32["lambda_f"] - r64["lambda_f"]) < 1e-5
+
+
+def test_poisson_graph_replays_match_eager(monkeypatch):
+    """The captured four-step CG block (replayed many times per solve, with the
+    host's convergence checks and copies between replays, and again across
+    solves on the same context) gives bit for bit the eager result; a whole
+    SCF too."""
+    with sq.Context(MG1, "Mg1", cell=True) as c:
+        f0 = c.cell_field("rho_atomic") + c.cell_field("b")
+        g = np.indices(f0.shape).astype(float)
+        pert = 0.01 * np.sin(2 * np.pi * g[2] / f0.shape[2]) * np.cos(2 * np.pi * g[0] / f0.shape[0])
+        fs = [f0, f0 + pert - pert.mean(), 2.0 * f0]
+        monkeypatch.setenv("SQ_NO_GRAPHS", "1")
+        eager = [c.poisson(f, tol=1e-12) for f in fs]
+        r_eager = c.scf(sq.SQ_NL_KB, sq.SQ_FP64, tol=1e-8, max_iter=200, mixing=0.3)
+        monkeypatch.delenv("SQ_NO_GRAPHS")
+        for _ in range(2):
+            for (pe, ie), f in zip(eager, fs):
+                pg, ig = c.poisson(f, tol=1e-12)
+                assert ig == ie and np.array_equal(pg, pe)
+        r_graph = c.scf(sq.SQ_NL_KB, sq.SQ_FP64, tol=1e-8, max_iter=200, mixing=0.3)
+    print("CG iterations per solve", [i for _, i in eager], flush=True)
+    assert r_graph["iterations"] == r_eager["iterations"] and r_graph["e_total"] == r_eager["e_total"]
