@@ -201,6 +201,8 @@ typedef struct {
     double exit_tol;         /* > 0: per-node early exit (sq_set_early_exit) from the second
                                 iteration on, at the previous iteration's Fermi level */
     int exit_min, exit_every;   /* its first check and spacing (<= 0: 30 and 5) */
+    double kerker;           /* > 0: Kerker-preconditioned mixing with this k0 (1/bohr), the new
+                                density (x - dX g) + beta P (f - dF g), P = q^2/(q^2 + k0^2) */
 } sq_scf_options;
 
 typedef struct {
@@ -310,6 +312,11 @@ int sq_mlp_loss_grad(sq_mlp *model, const float *x, const float *t, long n, doub
 /* initial density for the context's cell: max(0, rho_atomic + network),
  * scaled to N_e; rho0[(N0-1)(N1-1)(N2-1)] */
 int sq_mlp_guess(sq_mlp *model, sq_ctx *ctx, int bf16, double *rho0);
+
+/* The Kerker preconditioner of the mixing on a cell field: y = A (A + k0^2/2)^-1 u,
+ * A = -(1/2) Lap_h, by conjugate gradients (no reference counterpart: its mix
+ * is declared only, lsdft.h:183). */
+int sq_kerker(sq_ctx *ctx, const double *u, double k0, double *y, int *iterations);
 
 #ifdef __cplusplus
 }

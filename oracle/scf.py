@@ -87,6 +87,13 @@ def poisson_fft(coeffs, order, f):
     return np.real(np.fft.ifftn(2.0 * np.pi * fh / s))
 
 
+def kerker_fft(coeffs, order, u, k0):
+    """y = A (A + k0^2/2)^-1 u with A = -(1/2) Lap_h on the periodic cell, by
+    the stencil's exact symbol s(q): y_hat = s/(s + k0^2/2) u_hat (zero at q = 0)."""
+    s = laplacian_symbol(coeffs, order, u.shape)
+    return np.real(np.fft.ifftn(np.fft.fftn(u) * s / (s + 0.5 * k0 * k0)))
+
+
 def apply_laplacian(coeffs, order, x):
     """-(1/2) Lap_h x on the periodic cell, by the stencil (np.roll)."""
     y = (coeffs[0][0] + coeffs[1][0] + coeffs[2][0]) * x

@@ -28,7 +28,7 @@ EXPORTS = ["sq_last_error", "sq_default_options", "sq_create_from_deck", "sq_des
            "sq_get_image", "sq_default_scf_options", "sq_scf", "sq_get_cell_field", "sq_set_veff_cell", "sq_xc",
            "sq_poisson", "sq_set_early_exit", "sq_get_rule_sizes", "sq_forces", "sq_mlp_create", "sq_mlp_destroy",
            "sq_mlp_get_params", "sq_mlp_set_params", "sq_mlp_descriptors", "sq_mlp_add_sample", "sq_mlp_train",
-           "sq_mlp_forward", "sq_mlp_loss_grad", "sq_mlp_guess"]
+           "sq_mlp_forward", "sq_mlp_loss_grad", "sq_mlp_guess", "sq_kerker"]
 
 FIELDS = {"rho": 0, "rho_out": 1, "phi": 2, "vxc": 3, "veff": 4, "b": 5, "rho_atomic": 6}
 
@@ -48,7 +48,7 @@ class ScfOptions(C.Structure):
     _fields_ = [("nonlocal_mode", C.c_int), ("precision", C.c_int), ("max_iter", C.c_int), ("tol", C.c_double),
                 ("mixing", C.c_double), ("history", C.c_int), ("pulay_frequency", C.c_int),
                 ("tol_poisson", C.c_double), ("rho0", C.POINTER(C.c_double)), ("verbose", C.c_int),
-                ("exit_tol", C.c_double), ("exit_min", C.c_int), ("exit_every", C.c_int)]
+                ("exit_tol", C.c_double), ("exit_min", C.c_int), ("exit_every", C.c_int), ("kerker", C.c_double)]
 
 
 class ScfResult(C.Structure):
@@ -118,6 +118,7 @@ def lib():
     L.sq_set_early_exit.argtypes = [vp, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int]
     L.sq_get_rule_sizes.argtypes = [vp, ip]
     L.sq_forces.argtypes = [vp, dp, dp, dp, C.POINTER(ForceInfo)]
+    L.sq_kerker.argtypes = [vp, dp, C.c_double, dp, C.POINTER(C.c_int)]
     fp = C.POINTER(C.c_float)
     L.sq_mlp_create.argtypes = [C.c_int, C.c_uint, C.POINTER(vp)]
     L.sq_mlp_destroy.argtypes = [vp]
@@ -281,9 +282,11 @@ class Context:
         return n
 
     def scf(self, mode=SQ_NL_KB, precision=SQ_FP64, max_iter=0, tol=0.0, mixing=0.0, history=0,
-            pulay_frequency=0, tol_poisson=0.0, rho0=None, verbose=False, exit_tol=0.0, exit_min=0, exit_every=0):
+            pulay_frequency=0, tol_poisson=0.0, rho0=None, verbose=False, exit_tol=0.0, exit_min=0, exit_every=0,
+            kerker=0.0):
         o = ScfOptions()
         lib().sq_default_scf_options(C.byref(o))
+        o.kerker = float(kerker)
         o.exit_tol, o.exit_min, o.exit_every = float(exit_tol), int(exit_min), int(exit_every)
         o.nonlocal_mode, o.precision, o.max_iter = int(mode), int(precision), int(max_iter)
         o.tol, o.mixing, o.history, o.pulay_frequency = float(tol), float(mixing), int(history), int(pulay_frequency)
@@ -327,6 +330,14 @@ class Context:
         eps, v = np.zeros_like(r), np.zeros_like(r)
         _check(lib().sq_xc(self._h, _dp(r), r.size, _dp(eps), _dp(v)))
         return eps.reshape(np.shape(rho)), v.reshape(np.shape(rho))
+
+    def kerker(self, u, k0=1.0):
+        """The mixing's Kerker preconditioner applied to a cell field: (y, CG iterations)."""
+        ur = np.ascontiguousarray(u, np.float64).ravel()
+        y = np.zeros_like(ur)
+        its = C.c_int()
+        _check(lib().sq_kerker(self._h, _dp(ur), float(k0), _dp(y), C.byref(its)))
+        return y.reshape(np.shape(u)), its.value
 
     def poisson(self, f, tol=0.0):
         fr = np.ascontiguousarray(f, np.float64).ravel()

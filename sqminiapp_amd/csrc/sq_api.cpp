@@ -92,7 +92,8 @@ struct sq_ctx {
         int hist = 0;
         double *rho = nullptr, *rho_out = nullptr, *b = nullptr, *phi = nullptr, *eps = nullptr, *vxc = nullptr,
                *veff = nullptr, *r = nullptr, *p = nullptr, *Ap = nullptr, *f = nullptr, *x_prev = nullptr,
-               *f_prev = nullptr, *dX = nullptr, *dF = nullptr, *g = nullptr, *s = nullptr, *part = nullptr;
+               *f_prev = nullptr, *dX = nullptr, *dF = nullptr, *g = nullptr, *s = nullptr, *part = nullptr,
+               *ku = nullptr, *ky = nullptr;   // Kerker: averaged residual, its preconditioned form
         bool have_b = false;
         double eself = 0, b_scale = 0;
         std::vector<double> host_b;
@@ -128,7 +129,7 @@ void free_ctx(sq_ctx *c)
                     c->queue, c->nsteps};
     void *scf_ptrs[] = {c->scf.rho, c->scf.rho_out, c->scf.b, c->scf.phi, c->scf.eps, c->scf.vxc, c->scf.veff, c->scf.r,
                         c->scf.p, c->scf.Ap, c->scf.f, c->scf.x_prev, c->scf.f_prev, c->scf.dX, c->scf.dF, c->scf.g,
-                        c->scf.s, c->scf.part};
+                        c->scf.s, c->scf.part, c->scf.ku, c->scf.ky};
     if (c->scf.cg_exec) (void)hipGraphExecDestroy(c->scf.cg_exec);
     if (c->scf.cg_graph) (void)hipGraphDestroy(c->scf.cg_graph);
     for (void *p : scf_ptrs)
@@ -939,12 +940,13 @@ static int scf_alloc(sq_ctx *c, int hist)
     F.cg_exec = nullptr;
     F.cg_graph = nullptr;
     double **all[] = {&F.rho, &F.rho_out, &F.b, &F.phi, &F.eps, &F.vxc, &F.veff, &F.r, &F.p, &F.Ap, &F.f, &F.x_prev,
-                      &F.f_prev, &F.dX, &F.dF, &F.g, &F.s, &F.part};
+                      &F.f_prev, &F.dX, &F.dF, &F.g, &F.s, &F.part, &F.ku, &F.ky};
     for (double **q : all) {
         if (*q) (void)hipFree(*q);
         *q = nullptr;
     }
-    double **vecs[] = {&F.rho, &F.rho_out, &F.b, &F.phi, &F.eps, &F.vxc, &F.veff, &F.r, &F.p, &F.Ap, &F.f, &F.x_prev, &F.f_prev};
+    double **vecs[] = {&F.rho, &F.rho_out, &F.b, &F.phi, &F.eps, &F.vxc, &F.veff, &F.r, &F.p, &F.Ap, &F.f, &F.x_prev, &F.f_prev,
+                       &F.ku, &F.ky};
     for (double **v : vecs) {
         HIP_TRY(hipMalloc((void **)v, ng * sizeof(double)));
         HIP_TRY(hipMemset(*v, 0, ng * sizeof(double)));
@@ -1054,6 +1056,46 @@ static int scf_poisson(sq_ctx *c, const double *f, double tol, int max_it, int *
     if ((rc = scf_dot(c, F.phi, nullptr, 4, &mean))) return rc;
     mean /= (double)n;
     HIP_TRY(launch_axpbypgz(1.0, F.phi, 0.0, F.phi, -mean, nullptr, n, c->stream));
+    if (iters) *iters = it;
+    return SQ_OK;
+}
+
+// Kerker preconditioner y = A (A + k0^2/2)^-1 u, A = -(1/2) Lap_h on the cell
+// (symbol q^2 / (q^2 + k0^2)): (A + k0^2/2) y = A u by conjugate gradients from
+// y = 0 to ||r|| <= 1e-10 ||A u||, checked every 4 steps.  A u has zero mean,
+// so y does: a preconditioned residual moves no charge.  Uses the CG buffers
+// of the Poisson solve (not in use meanwhile) and runs eagerly.
+static int scf_kerker(sq_ctx *c, const double *u, double *y, double k0, int *iters)
+{
+    auto &F = c->scf;
+    const System &S = c->sys;
+    int G[3];
+    if (int rc = cell_dims(c, G)) return rc;
+    const StencilCoeffs sc = stencil_of(S);
+    const long n = F.ng;
+    const double shift = 0.5 * k0 * k0;
+    int rc;
+    HIP_TRY(launch_lap_periodic(u, F.r, G, S.order, sc, c->stream));       // r = A u - (A + s) 0
+    HIP_TRY(hipMemsetAsync(y, 0, n * sizeof(double), c->stream));
+    HIP_TRY(hipMemcpyAsync(F.p, F.r, n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    double bnorm2 = 0.0;
+    if ((rc = scf_dot(c, F.r, F.r, 0, &bnorm2))) return rc;
+    const double stop = 1e-20 * bnorm2;
+    double rr = bnorm2;
+    int it = 0;
+    while (rr > stop && it < 2000) {
+        for (int k = 0; k < 4; k++, it++) {
+            HIP_TRY(launch_lap_periodic(F.p, F.Ap, G, S.order, sc, c->stream));
+            HIP_TRY(launch_axpbypgz(shift, F.p, 1.0, F.Ap, 0.0, nullptr, n, c->stream));   // (A + s) p
+            if ((rc = scf_dot(c, F.p, F.Ap, 1, nullptr))) return rc;
+            HIP_TRY(launch_cg_xr(y, F.r, F.p, F.Ap, F.s, n, c->stream));
+            if ((rc = scf_dot(c, F.r, F.r, 2, nullptr))) return rc;
+            HIP_TRY(launch_cg_p(F.p, F.r, F.s, n, c->stream));
+        }
+        HIP_TRY(hipMemcpyAsync(&rr, F.s, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(hipStreamSynchronize(c->stream));
+    }
+    if (!(rr <= stop)) return set_err(SQ_ERR_RANGE, "Kerker: conjugate gradients did not converge");
     if (iters) *iters = it;
     return SQ_OK;
 }
@@ -1223,7 +1265,14 @@ int sq_scf(sq_ctx *c, const sq_scf_options *opt_in, sq_scf_result *res)
                 mu = nhist;
             }
         }
-        HIP_TRY(launch_pulay_update(F.rho, F.f, F.dX, F.dF, F.g, mu, beta, n, F.rho_out, c->stream));
+        if (opt.kerker > 0) {
+            // x_new = (x - dX g) + beta P (f - dF g)
+            HIP_TRY(launch_pulay_split(F.rho, F.f, F.dX, F.dF, F.g, mu, n, F.rho_out, F.ku, c->stream));
+            if ((rc = scf_kerker(c, F.ku, F.ky, opt.kerker, nullptr))) return rc;
+            HIP_TRY(launch_axpbypgz(beta, F.ky, 1.0, F.rho_out, 0.0, nullptr, n, c->stream));
+        } else {
+            HIP_TRY(launch_pulay_update(F.rho, F.f, F.dX, F.dF, F.g, mu, beta, n, F.rho_out, c->stream));
+        }
         std::swap(F.rho, F.rho_out);   // rho_out now holds the mixed density; keep rho as the input
     }
     HIP_TRY(hipStreamSynchronize(c->stream));
@@ -1850,6 +1899,22 @@ int sq_poisson(sq_ctx *c, const double *f, double tol, double *phi, int *iters)
     HIP_TRY(hipMemset(F.phi, 0, F.ng * sizeof(double)));
     if ((rc = scf_poisson(c, F.eps, tol > 0 ? tol : c->sys.tol_poisson, 20000, iters))) return rc;
     HIP_TRY(hipMemcpy(phi, F.phi, F.ng * sizeof(double), hipMemcpyDeviceToHost));
+    return SQ_OK;
+}
+
+int sq_kerker(sq_ctx *c, const double *u, double k0, double *y, int *iters)
+{
+    if (!c || !u || !y) return set_err(SQ_ERR_ARG, "null argument");
+    if (!(k0 > 0)) return set_err(SQ_ERR_ARG, "k0 must be positive");
+    if (!c->stream) return set_err(SQ_ERR_STATE, "host-only context (created with device < 0)");
+    int G[3], rc;
+    if ((rc = cell_dims(c, G))) return rc;
+    HIP_TRY(hipSetDevice(c->device));
+    if ((rc = scf_alloc(c, std::max(1, c->scf.hist)))) return rc;
+    auto &F = c->scf;
+    HIP_TRY(hipMemcpy(F.ku, u, F.ng * sizeof(double), hipMemcpyHostToDevice));
+    if ((rc = scf_kerker(c, F.ku, F.ky, k0, iters))) return rc;
+    HIP_TRY(hipMemcpy(y, F.ky, F.ng * sizeof(double), hipMemcpyDeviceToHost));
     return SQ_OK;
 }
 

@@ -138,6 +138,8 @@ hipError_t launch_cg_xr(double *x, double *r, const double *p, const double *Ap,
 hipError_t launch_cg_p(double *p, const double *r, double *s, long n, hipStream_t st);
 hipError_t launch_pulay_update(const double *x, const double *f, const double *dX, const double *dF, const double *g, int m,
                                double beta, long n, double *xn, hipStream_t st);
+hipError_t launch_pulay_split(const double *x, const double *f, const double *dX, const double *dF, const double *g, int m,
+                              long n, double *xa, double *u, hipStream_t st);
 hipError_t launch_veff_to_diag(const double *veff, double cdiag, const int ext0[3], const int extn[3], const int G[3],
                                double *d64, float *d32, hipStream_t st);
 

@@ -152,6 +152,25 @@ __global__ void __launch_bounds__(kT) pulay_update(const double *__restrict__ x,
     }
 }
 
+// The two halves of a Pulay step for a preconditioned residual:
+// xa = x - sum_j g_j dX_j (the averaged iterate), u = f - sum_j g_j dF_j (the
+// averaged residual); the new iterate is xa + beta P u.
+__global__ void __launch_bounds__(kT) pulay_split(const double *__restrict__ x, const double *__restrict__ f,
+                                                  const double *__restrict__ dX, const double *__restrict__ dF,
+                                                  const double *__restrict__ g, int m, long n, double *__restrict__ xa,
+                                                  double *__restrict__ u)
+{
+    for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) {
+        double a = x[q], b = f[q];
+        for (int j = 0; j < m; j++) {
+            a -= g[j] * dX[(long)j * n + q];
+            b -= g[j] * dF[(long)j * n + q];
+        }
+        xa[q] = a;
+        u[q] = b;
+    }
+}
+
 // cube diagonal c_diag + V_eff over the extended node box; V_eff is a field on
 // the periodic cell G, wrapped with period G in every direction.
 __global__ void __launch_bounds__(kT) veff_to_diag(const double *__restrict__ veff, double cdiag, int e0x, int e0y, int e0z,
@@ -216,6 +235,13 @@ hipError_t launch_pulay_update(const double *x, const double *f, const double *d
                                double beta, long n, double *xn, hipStream_t st)
 {
     hipLaunchKernelGGL(pulay_update, grid_for(n), dim3(kT), 0, st, x, f, dX, dF, g, m, beta, n, xn);
+    return hipGetLastError();
+}
+
+hipError_t launch_pulay_split(const double *x, const double *f, const double *dX, const double *dF, const double *g, int m,
+                              long n, double *xa, double *u, hipStream_t st)
+{
+    hipLaunchKernelGGL(pulay_split, grid_for(n), dim3(kT), 0, st, x, f, dX, dF, g, m, n, xa, u);
     return hipGetLastError();
 }
 

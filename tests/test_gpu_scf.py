@@ -122,3 +122,20 @@ def test_poisson_graph_replays_match_eager(monkeypatch):
         r_graph = c.scf(sq.SQ_NL_KB, sq.SQ_FP64, tol=1e-8, max_iter=200, mixing=0.3)
     print("CG iterations per solve", [i for _, i in eager], flush=True)
     assert r_graph["iterations"] == r_eager["iterations"] and r_graph["e_total"] == r_eager["e_total"]
+
+
+def test_kerker_matches_fft_and_scf_reaches_the_same_energy(mg1_cell_oracle):
+    I = mg1_cell_oracle.info()
+    coeffs = [list(I.coeffs[d]) for d in range(3)]
+    with sq.Context(MG1, "Mg1", cell=True) as c:
+        rng = np.random.default_rng(3)
+        u = rng.standard_normal(c.cell_shape())
+        y, its = c.kerker(u, 1.0)
+        ref = OS.kerker_fft(coeffs, I.order, u, 1.0)
+        print("Kerker CG iterations", its, flush=True)
+        assert rel_err(y, ref) < 1e-8 and abs(y.mean()) < 1e-12
+        r0 = c.scf(sq.SQ_NL_KB, sq.SQ_FP64, tol=1e-8, max_iter=200, mixing=0.3)
+        rk = c.scf(sq.SQ_NL_KB, sq.SQ_FP64, tol=1e-8, max_iter=200, mixing=0.3, kerker=1.0)
+    print(f"Mg1 SCF iterations: Pulay {r0['iterations']}, Kerker-Pulay {rk['iterations']}", flush=True)
+    assert r0["converged"] and rk["converged"]
+    assert abs(rk["e_total"] - r0["e_total"]) / I.nAtoms < 1e-8

@@ -184,3 +184,23 @@ def test_oracle_scf_converges_on_a_small_cube_deck(tmp_path):
     assert out["rho_out"].sum() * I.dV == pytest.approx(8.0, rel=1e-10)
     # self-consistency: the last input density reproduces itself
     assert np.linalg.norm(out["rho_out"] - out["rho_in"]) / np.linalg.norm(out["rho_out"]) < 1e-6
+
+
+def test_kerker_symbol_on_plane_waves(mg1_oracle):
+    """kerker_fft multiplies a lattice plane wave by s/(s + k0^2/2), s the
+    stencil's symbol at its wavevector, and removes the mean."""
+    coeffs, order = _coeffs(mg1_oracle)
+    shape = (16, 17, 10)
+    g = np.indices(shape).astype(float)
+    k = (2, 3, 1)
+    wave = np.cos(2 * np.pi * (k[0] * g[0] / shape[0] + k[1] * g[1] / shape[1] + k[2] * g[2] / shape[2]))
+    sym = sum(coeffs[d][0] + sum(2 * coeffs[d][a] * np.cos(2 * np.pi * a * kk / n) for a in range(1, order + 1))
+              for d, kk, n in ((2, k[0], shape[0]), (1, k[1], shape[1]), (0, k[2], shape[2])))
+    y = OS.kerker_fft(coeffs, order, wave + 3.0, 0.8)
+    assert np.max(np.abs(y - sym / (sym + 0.32) * wave)) < 1e-12
+    # and agrees with the stencil applied to the FFT Helmholtz solution
+    rng = np.random.default_rng(1)
+    u = rng.standard_normal(shape)
+    y = OS.kerker_fft(coeffs, order, u, 1.0)
+    Ay = OS.apply_laplacian(coeffs, order, y) + 0.5 * y
+    assert np.max(np.abs(Ay - OS.apply_laplacian(coeffs, order, u))) < 1e-10 * np.max(np.abs(u))
