@@ -1710,19 +1710,31 @@ int sq_mlp_add_sample(sq_mlp *m, sq_ctx *c)
     return SQ_OK;
 }
 
+// Activations, deltas and partial sums of one pass over N rows.
+struct MlpScratch {
+    float *H1 = nullptr, *H2 = nullptr, *Y = nullptr, *D1 = nullptr, *D2 = nullptr, *D3 = nullptr;
+    double *part = nullptr, *lsum = nullptr;
+    int nchunk = 1;
+    int alloc(DevBufs &B, long N, int H)
+    {
+        nchunk = (int)std::max<long>(1, std::min<long>(128, N / 256));
+        int rc;
+        if ((rc = B.get(&H1, N * H)) || (rc = B.get(&H2, N * H)) || (rc = B.get(&Y, N)) || (rc = B.get(&D1, N * H)) ||
+            (rc = B.get(&D2, N * H)) || (rc = B.get(&D3, N)) || (rc = B.get(&part, (size_t)nchunk * H * H)) ||
+            (rc = B.get(&lsum, 1)))
+            return rc;
+        return SQ_OK;
+    }
+};
+
 // L and dL/dp (double) of the current parameters on normalized data already on
-// the device (X: N x K, T: N), with the activations and deltas in scratch
-static int mlp_loss_grad_dev(sq_mlp *m, const float *X, const float *T, long N, double *loss, DevBufs &B)
+// the device (X: N x K, T: N), with the activations and deltas in the scratch W
+static int mlp_loss_grad_dev(sq_mlp *m, const float *X, const float *T, long N, double *loss, const MlpScratch &W)
 {
     const int K = m->K, H = m->H;
-    float *H1, *H2, *Y, *D1, *D2, *D3;
-    double *part, *lsum;
-    const int nchunk = (int)std::max<long>(1, std::min<long>(128, N / 256));
-    int rc;
-    if ((rc = B.get(&H1, N * H)) || (rc = B.get(&H2, N * H)) || (rc = B.get(&Y, N)) || (rc = B.get(&D1, N * H)) ||
-        (rc = B.get(&D2, N * H)) || (rc = B.get(&D3, N)) || (rc = B.get(&part, (size_t)nchunk * H * H)) ||
-        (rc = B.get(&lsum, 1)))
-        return rc;
+    float *H1 = W.H1, *H2 = W.H2, *Y = W.Y, *D1 = W.D1, *D2 = W.D2, *D3 = W.D3;
+    double *part = W.part, *lsum = W.lsum;
+    const int nchunk = W.nchunk;
     HIP_TRY(launch_mlp_forward(X, N, m->p, nullptr, nullptr, H1, H2, Y, 0, m->stream));
     HIP_TRY(launch_mlp_deltas(H1, H2, Y, T, N, m->p, D1, D2, D3, m->stream));
     double *g = m->g;
@@ -1752,7 +1764,8 @@ int sq_mlp_loss_grad(sq_mlp *m, const float *X, const float *T, long N, double *
     if ((rc = B.get(&dX, N * m->K)) || (rc = B.get(&dT, N))) return rc;
     HIP_TRY(hipMemcpy(dX, X, N * m->K * sizeof(float), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(dT, T, N * sizeof(float), hipMemcpyHostToDevice));
-    if ((rc = mlp_loss_grad_dev(m, dX, dT, N, loss, B))) return rc;
+    MlpScratch W;
+    if ((rc = W.alloc(B, N, m->H)) || (rc = mlp_loss_grad_dev(m, dX, dT, N, loss, W))) return rc;
     if (grad) HIP_TRY(hipMemcpy(grad, m->g, m->np * sizeof(double), hipMemcpyDeviceToHost));
     HIP_TRY(hipStreamSynchronize(m->stream));
     return SQ_OK;
@@ -1808,16 +1821,14 @@ int sq_mlp_train(sq_mlp *m, int epochs, double lr, double *loss)
     if ((rc = B.get(&dX, N * K)) || (rc = B.get(&dT, N))) return rc;
     HIP_TRY(hipMemcpy(dX, Xn.data(), N * K * sizeof(float), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(dT, Tn.data(), N * sizeof(float), hipMemcpyHostToDevice));
+    MlpScratch W;
+    if ((rc = W.alloc(B, N, m->H))) return rc;
     for (int e = 0; e < epochs; e++) {
-        DevBufs S;
-        if ((rc = mlp_loss_grad_dev(m, dX, dT, N, nullptr, S))) return rc;
+        if ((rc = mlp_loss_grad_dev(m, dX, dT, N, nullptr, W))) return rc;
         HIP_TRY(launch_adam(m->p, m->g, m->m, m->v, m->np, (float)lr, ++m->t, m->stream));
         if ((e & 63) == 63) HIP_TRY(hipStreamSynchronize(m->stream));
     }
-    if (loss) {
-        DevBufs S;
-        if ((rc = mlp_loss_grad_dev(m, dX, dT, N, loss, S))) return rc;
-    }
+    if (loss && (rc = mlp_loss_grad_dev(m, dX, dT, N, loss, W))) return rc;
     HIP_TRY(hipStreamSynchronize(m->stream));
     return SQ_OK;
 }
