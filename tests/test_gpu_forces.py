@@ -112,3 +112,27 @@ def test_fp32_forces_against_fp64(tmp_path):
     diff = np.max(np.abs(F32 - F64))
     print(f"max |F32 - F64| = {diff:.3e} Ha/bohr (FP32 SCF {r32['iterations']} it)", flush=True)
     assert diff < 1e-4
+
+
+def test_force_and_network_error_paths(tmp_path):
+    """The new entry points fail loudly, with the documented status."""
+    with sq.Context(MG1, "Mg1", cell=True) as c:
+        with pytest.raises(sq.SQError) as e:
+            c.forces()                                   # no SCF yet
+        assert e.value.code == 9
+        r = c.scf(sq.SQ_NL_REF_COLLAPSED, sq.SQ_FP64, tol=1e-4, max_iter=100, mixing=0.3)
+        assert r["iterations"] > 0
+        with pytest.raises(sq.SQError) as e:
+            c.forces()                                   # the collapsed operator has no forces here
+        assert e.value.code == 4
+        with pytest.raises(sq.SQError) as e:
+            c.kerker(np.zeros(c.cell_shape()), 0.0)
+        assert e.value.code == 4
+    with sq.Mlp() as m:
+        with pytest.raises(sq.SQError) as e:
+            m.train(10)                                  # no samples
+        assert e.value.code == 9
+    with sq.Context(MG1, "Mg1") as c:                    # node box is not the cell
+        with pytest.raises(sq.SQError) as e:
+            c.descriptors()
+        assert e.value.code == 9
