@@ -54,9 +54,9 @@ KERKER_K0 = 1.0
 
 def scf_wall_time(d, name, max_iter, mode, verbose=False):
     """SCF wall time on the periodic cell of the same deck: plain FP64 to a
-    density residual of 1e-6, and the FP32 form to 1e-5, its energy judged
-    against FP64 (the metric's 1e-5 Ha/atom); then the forces of each form
-    (the metric's 1e-4 Ha/bohr), their time reported beside the SCF's."""
+    density residual of 1e-6, and the FP32 form to 1e-5, its energy compared with
+    FP64 (within 1e-5 Ha/atom); then the forces of each form (within 1e-4
+    Ha/bohr), their time reported beside the SCF's; then the chosen form."""
     import sqminiapp_amd as sq
     from sqminiapp_amd import train_density_net
     with sq.Context(d, name, device=0, cell=True) as c:

@@ -88,7 +88,7 @@ def test_scf_small_cube_matches_oracle_scf(tmp_path):
 
 def test_scf_mg1_fp64_and_fp32_forms():
     """The FP32 recurrence leaves ~1e-6 relative noise in rho, so the FP32 form
-    stops at a density residual of 1e-5; its energy is judged against FP64."""
+    stops at a density residual of 1e-5; its energy is compared with FP64."""
     with sq.Context(MG1, "Mg1", cell=True) as c:
         r64 = c.scf(sq.SQ_NL_KB, sq.SQ_FP64, tol=1e-7, max_iter=200, mixing=0.3)
         r32 = c.scf(sq.SQ_NL_KB, sq.SQ_FP32, tol=1e-5, max_iter=200, mixing=0.3)
