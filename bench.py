@@ -45,7 +45,34 @@ def cpu_baseline(deck_dir, name, n_sample, threads):
     return dict(value=n_sample / dt, unit="grid-nodes/s", cores=threads, kind="port",
                 sample=f"{n_sample} evenly spaced nodes of the same deck, KB mode, FP64, oracle/sq_oracle.c "
                        f"(per-node operator rebuild, AIJ SpMV, projector scan over all images, full-z QL), "
-                       f"{dt:.1f} s on {threads} threads")
+                       f"{dt:.1f} s on {threads} threads", host=_cpu_model())
+
+
+def cpu_optimised(deck_dir, name, n_sample, threads):
+    """The optimised CPU form of the same pass (oracle/sq_cpu_fast.c: matrix-free
+    stencil, per-node image lists, AVX2/FMA, OpenMP), on a larger sample."""
+    from oracle import oracle as O
+    O.build()
+    S = O.System(deck_dir, name)
+    S.build_nonlocal()
+    total = S.box[3] * S.box[4] * S.box[5]
+    ids = np.linspace(0, total - 1, n_sample).astype(np.int32)
+    t0 = time.perf_counter()
+    S.fast_nodes_and_weights(node_ids=ids, nthreads=threads)
+    dt = time.perf_counter() - t0
+    return dict(value=n_sample / dt, unit="grid-nodes/s", cores=threads, kind="port",
+                sample=f"{n_sample} evenly spaced nodes of the same deck, KB mode, FP64, oracle/sq_cpu_fast.c, "
+                       f"{dt:.1f} s on {threads} threads", host=_cpu_model())
+
+
+def _cpu_model():
+    try:
+        for ln in open("/proc/cpuinfo"):
+            if ln.startswith("model name"):
+                return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 KERNEL_NAMES = {0: "lanczos_kernel (generic)", 1: "lanczos_zmarch (column-marching)"}
@@ -202,6 +229,7 @@ def main():
     if not args.no_cpu_baseline:
         threads = max(1, min(16, os.cpu_count() or 1))
         out["cpu_baseline"] = cpu_baseline(d, name, args.cpu_sample, threads)
+        out["cpu_optimised"] = cpu_optimised(d, name, 8 * args.cpu_sample, threads)
     ctx.close()
     print(json.dumps(out), flush=True)
 

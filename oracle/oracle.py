@@ -71,6 +71,7 @@ def lib():
         L.sqo_lanczos.argtypes = [vp, C.c_int, dp, C.c_double, C.c_int, C.c_int, C.c_int, dp, dp]
         L.sqo_tridiag_nodes_weights.argtypes = [dp, dp, C.c_int, dp, dp, dp, dp]
         L.sqo_nodes_and_weights.argtypes = [vp, C.c_int, dp, C.c_double, ip, C.c_int, dp, dp, dp, dp, C.c_int]
+        L.sqo_fast_nodes_and_weights.argtypes = [vp, dp, C.c_double, ip, C.c_int, dp, dp, dp, dp, C.c_int]
         L.sqo_density_fermi_energy.argtypes = [dp, dp, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
                                                dp, dp, dp, dp]
         L.sqo_electron_count.argtypes = [dp, dp, C.c_long, C.c_double, C.c_double]
@@ -359,6 +360,24 @@ class System:
         field, const = self._veff(veff)
         _check(lib().sqo_nodes_and_weights(self._h, mode, _dp(field), const, _ip(ids), n, _dp(lam), _dp(w),
                                            _dp(lmin), _dp(lmax), int(nthreads)), "nodes_and_weights")
+        return lam, w, lmin, lmax
+
+    def fast_nodes_and_weights(self, veff=None, node_ids=None, nthreads=1):
+        """The optimised CPU form (oracle/sq_cpu_fast.c) of nodes_and_weights
+        with the Kleinman-Bylander operator: matrix-free stencil, per-node
+        image lists, FMA; equal to the restatement to rounding."""
+        I = self.info()
+        box = self.box
+        total = box[3] * box[4] * box[5]
+        ids = None if node_ids is None else np.ascontiguousarray(node_ids, np.int32)
+        n = total if ids is None else len(ids)
+        lam = np.zeros((n, I.nqp))
+        w = np.zeros((n, I.nqp))
+        lmin = np.zeros(n)
+        lmax = np.zeros(n)
+        field, const = self._veff(veff)
+        _check(lib().sqo_fast_nodes_and_weights(self._h, _dp(field), const, _ip(ids), n, _dp(lam), _dp(w),
+                                                _dp(lmin), _dp(lmax), int(nthreads)), "fast_nodes_and_weights")
         return lam, w, lmin, lmax
 
     # ---- forces (the derivative of the SCF energy; see sq_oracle.c)
