@@ -79,30 +79,41 @@ KERNEL_NAMES = {0: "lanczos_kernel (generic)", 1: "lanczos_zmarch (column-marchi
 KERKER_K0 = 1.0
 
 
+SCF_TOL = 1e-5        # the stopping tolerance of every timed form (the FP32 floor is about 5e-6)
+SCF_TOL_REF = 1e-6    # the accuracy yardstick: plain FP64 converged further
+
+
 def scf_wall_time(d, name, max_iter, mode, verbose=False):
-    """SCF wall time on the periodic cell of the same deck: plain FP64 to a
-    density residual of 1e-6, and the FP32 form to 1e-5, its energy compared with
-    FP64 (within 1e-5 Ha/atom); then the forces of each form (within 1e-4
-    Ha/bohr), their time reported beside the SCF's; then the chosen form."""
+    """SCF wall time on the periodic cell of the same deck.
+
+    The yardstick is plain FP64 (the deck's mixing, superposed atoms) converged
+    to a density residual of 1e-6; its energy and forces are what every other
+    form is compared with (targets: 1e-5 Ha/atom, 1e-4 Ha/bohr).  Every timed
+    form stops at the same residual, 1e-5: plain FP64, FP32, FP64 with Kerker
+    mixing, and the chosen form (FP32, Kerker, learned initial density).  The
+    speed-ups are wall-time ratios at that common tolerance."""
     import sqminiapp_amd as sq
     from sqminiapp_amd import train_density_net
+    kb = mode == sq.SQ_NL_KB
     with sq.Context(d, name, device=0, cell=True) as c:
         I = c.info
-        r64 = c.scf(mode, sq.SQ_FP64, tol=1e-6, max_iter=max_iter, verbose=verbose)
-        F64, _, _, fi64 = c.forces() if mode == sq.SQ_NL_KB else (None, None, None, None)
-        r32 = c.scf(mode, sq.SQ_FP32, tol=1e-5, max_iter=max_iter, verbose=verbose)
-        F32, _, _, fi32 = c.forces() if mode == sq.SQ_NL_KB else (None, None, None, None)
+        ref = c.scf(mode, sq.SQ_FP64, tol=SCF_TOL_REF, max_iter=max_iter, verbose=verbose)
+        Fref, _, _, firef = c.forces() if kb else (None, None, None, None)
+        r64 = c.scf(mode, sq.SQ_FP64, tol=SCF_TOL, max_iter=max_iter, verbose=verbose)
+        r32 = c.scf(mode, sq.SQ_FP32, tol=SCF_TOL, max_iter=max_iter, verbose=verbose)
+        F32, _, _, fi32 = c.forces() if kb else (None, None, None, None)
         # the chosen form: FP32, Kerker-preconditioned mixing (k0 = 1/bohr, the deck's mixing
         # parameter), from the learned initial density (weights trained on other cells);
         # and plain FP64 with the same mixing, for the precision-only ratio
-        rk = c.scf(mode, sq.SQ_FP64, tol=1e-6, max_iter=max_iter, kerker=KERKER_K0, verbose=verbose)
-        net = train_density_net.load() if mode == sq.SQ_NL_KB else None
+        rk = c.scf(mode, sq.SQ_FP64, tol=SCF_TOL, max_iter=max_iter, kerker=KERKER_K0, verbose=verbose)
+        net = train_density_net.load() if kb else None
         rl = Fl = None
         if net is not None:
             t0 = time.perf_counter()
             guess = net.guess(c, bf16=True)
             t_guess = time.perf_counter() - t0
-            rl = c.scf(mode, sq.SQ_FP32, tol=1e-5, max_iter=max_iter, rho0=guess, kerker=KERKER_K0, verbose=verbose)
+            rl = c.scf(mode, sq.SQ_FP32, tol=SCF_TOL, max_iter=max_iter, rho0=guess, kerker=KERKER_K0,
+                       verbose=verbose)
             Fl, _, _, fil = c.forces()
             atomic = c.cell_field("rho_atomic")
             atomic = atomic * (I.noetot / (atomic.sum() * I.dV))
@@ -111,26 +122,27 @@ def scf_wall_time(d, name, max_iter, mode, verbose=False):
                    float(np.linalg.norm(guess - rho) / np.linalg.norm(rho)))
             net.close()
     keep = ("iterations", "converged", "residual", "wall_s", "sq_s", "poisson_s", "e_total", "lambda_f")
-    f64 = {k: r64[k] for k in keep}
-    f32 = {k: r32[k] for k in keep}
+    de = lambda r: abs(r["e_total"] - ref["e_total"]) / I.nAtoms
     out = {"workload": f"periodic cell {I.rank_box[3]}x{I.rank_box[4]}x{I.rank_box[5]} = {I.n_nodes} nodes, "
                        f"BETA {I.beta:g}, deck mixing (Periodic Pulay), LDA-PW, start: superposed atoms",
-           "plain_fp64": f64, "fp32_form": f32,
-           "d_energy_per_atom": abs(r32["e_total"] - r64["e_total"]) / I.nAtoms,
+           "tol": SCF_TOL,
+           "reference_fp64": {"tol": SCF_TOL_REF, **{k: ref[k] for k in keep}},
+           "plain_fp64": {**{k: r64[k] for k in keep}, "d_energy_per_atom": de(r64)},
+           "fp32_form": {**{k: r32[k] for k in keep}, "d_energy_per_atom": de(r32)},
            "wall_speedup_fp32": r64["wall_s"] / r32["wall_s"] if r32["wall_s"] > 0 else None}
-    if F64 is not None:
-        out["forces"] = {"fp64_s": fi64["wall_s"], "fp32_s": fi32["wall_s"], "nodes": fi64["nodes"],
-                         "max_abs_fp64": float(abs(F64).max()), "max_diff_fp32_fp64": float(abs(F32 - F64).max()),
-                         "unit": "Ha/bohr"}
-    out["plain_fp64_kerker"] = {k: rk[k] for k in keep}
+    if Fref is not None:
+        out["forces"] = {"reference_fp64_s": firef["wall_s"], "fp32_s": fi32["wall_s"], "nodes": firef["nodes"],
+                         "max_abs_reference": float(abs(Fref).max()),
+                         "max_diff_fp32_vs_reference": float(abs(F32 - Fref).max()), "unit": "Ha/bohr"}
+    out["plain_fp64_kerker"] = {**{k: rk[k] for k in keep}, "d_energy_per_atom": de(rk)}
     if rl is not None:
         wall = rl["wall_s"] + t_guess
         out["chosen_form"] = {"form": "FP32 Lanczos, Kerker-preconditioned Pulay mixing (k0 = 1/bohr), from the learned "
                                       "initial density (BF16 network inference)",
                               **{k: rl[k] for k in keep}, "guess_s": t_guess, "wall_with_guess_s": wall,
                               "initial_error_atoms": err[0], "initial_error_network": err[1],
-                              "d_energy_per_atom": abs(rl["e_total"] - r64["e_total"]) / I.nAtoms,
-                              "max_force_diff_vs_fp64": float(abs(Fl - F64).max()),
+                              "d_energy_per_atom": de(rl),
+                              "max_force_diff_vs_reference": float(abs(Fl - Fref).max()),
                               "wall_speedup_vs_fp64": r64["wall_s"] / wall,
                               "wall_speedup_vs_fp64_kerker": rk["wall_s"] / wall}
     return out
