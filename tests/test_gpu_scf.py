@@ -1,7 +1,8 @@
 """The SCF loop on the GPU (include/sq_amd.h, SCF section) against the oracle:
 each V_eff-pipeline piece on its own (LDA, Poisson, V_eff on the cell), then
 whole SCF runs: a small-cube deck against the oracle SCF with the same mixing,
-and the Mg1 deck in the FP64 and FP32 forms.
+the Mg1 deck in the FP64 and FP32 forms, and the chosen form (FP32 with Kerker
+mixing) against the energy and force targets.
 
 Tolerances: LDA 1e-13 relative (same formulas, device libm); Poisson 1e-8
 relative (conjugate gradients to 1e-12 against the FFT solve); Lanczos nodes
@@ -139,3 +140,22 @@ def test_kerker_matches_fft_and_scf_reaches_the_same_energy(mg1_cell_oracle):
     print(f"Mg1 SCF iterations: Pulay {r0['iterations']}, Kerker-Pulay {rk['iterations']}", flush=True)
     assert r0["converged"] and rk["converged"]
     assert abs(rk["e_total"] - r0["e_total"]) / I.nAtoms < 1e-8
+
+
+def test_chosen_form_meets_the_accuracy_targets():
+    """FP32 Lanczos with Kerker mixing (k0 = 1/bohr, mixing 0.7), stopped at a
+    density residual of 1e-5, against plain FP64 with the deck's mixing
+    converged to 1e-7: energy within 1e-5 Ha/atom, forces within 1e-4 Ha/bohr,
+    in fewer iterations than plain FP64 stopped at the same residual."""
+    with sq.Context(MG1, "Mg1", cell=True) as c:
+        n = c.info.nAtoms
+        ref = c.scf(sq.SQ_NL_KB, sq.SQ_FP64, tol=1e-7, max_iter=200, mixing=0.3)
+        Fref = c.forces()[0]
+        plain = c.scf(sq.SQ_NL_KB, sq.SQ_FP64, tol=1e-5, max_iter=200, mixing=0.3)
+        chosen = c.scf(sq.SQ_NL_KB, sq.SQ_FP32, tol=1e-5, max_iter=200, kerker=1.0, mixing=0.7)
+        F = c.forces()[0]
+    print(f"Mg1: plain FP64 {plain['iterations']} iterations, chosen form {chosen['iterations']}", flush=True)
+    assert ref["converged"] and plain["converged"] and chosen["converged"]
+    assert abs(chosen["e_total"] - ref["e_total"]) / n < 1e-5
+    assert np.max(np.abs(F - Fref)) < 1e-4
+    assert chosen["iterations"] < plain["iterations"]
