@@ -508,6 +508,9 @@ int finish_device_setup(sq_ctx *c)
         int best = 0;
         if (fixed) best = atoi(fixed) == 1024 && grids[prec][1] > 0 ? 1 : 0;
         else if (grids[prec][1] > 0) {
+            // each probe runs one node per workgroup of its shape's full grid; the
+            // shapes' grids differ when their occupancies do, so compare time per node
+            double per_node[2] = {0.0, 0.0};
             for (int sh = 0; sh < 2; sh++) {
                 KParams P = make_params(c, SQ_NL_KB, prec, 0, std::min<long>(grids[prec][sh], S.n_nodes()), true);
                 P.zwg = sh ? 1024 : 512;
@@ -522,8 +525,9 @@ int finish_device_setup(sq_ctx *c)
                 }
                 HIP_TRY(hipEventSynchronize(c->ev[1]));
                 HIP_TRY(hipEventElapsedTime(&c->tune_ms[prec][sh], c->ev[0], c->ev[1]));
+                per_node[sh] = c->tune_ms[prec][sh] / (double)P.node_count;
             }
-            best = c->tune_ms[prec][1] < c->tune_ms[prec][0] ? 1 : 0;
+            best = per_node[1] < per_node[0] ? 1 : 0;
         }
         c->zwg[prec] = best ? 1024 : 512;
         c->zgrid[prec] = grids[prec][best];

@@ -55,8 +55,13 @@ __device__ __forceinline__ void sd_channels(const SdColumn<T> &c, T t0, T t2, T 
     for (int ch = 6; ch < NLM; ch++) out[ch] = T(0);
 }
 
+// The FP32 form of the 512-thread shape is held to 128 VGPRs (four waves per
+// SIMD), so two workgroups, two nodes, share a CU and cover each other's
+// barrier and load waits (804 against 867 ms for the 16-atom pass).  The FP64
+// form would spill three times as much there and keeps 256 VGPRs.
 template <typename T, int NLM, int CPT, int MW, int LAYOUT, int WG>
-__global__ void __launch_bounds__(WG) lanczos_zmarch(KParams P)
+__global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(WG == 512 && sizeof(T) == 4 ? 4 : 1)))
+lanczos_zmarch(KParams P)
 {
     constexpr int NW = WG / 64;
     constexpr int O = 6;
