@@ -66,9 +66,11 @@ lanczos_zmarch(KParams P)
     constexpr int NW = WG / 64;
     constexpr int O = 6;
     // z-planes per trip of the scatter (U) and gather (UG) loops: as many loads
-    // in flight as the register budget of two waves per SIMD allows
+    // in flight as the register budget allows (two waves per SIMD for the FP64
+    // 512-thread shape; four for the others, where the FP32 512-thread shape
+    // measured 804, 781 and 767 ms at 16, 8 and 4 planes)
     constexpr bool WL = WG == 1024;   // register-lean shape: 4 waves per SIMD
-    constexpr int U = WL ? (sizeof(T) == 8 ? 4 : 8) : (sizeof(T) == 8 ? 8 : 16);
+    constexpr int U = WL ? (sizeof(T) == 8 ? 4 : 8) : (sizeof(T) == 8 ? 8 : 4);
     constexpr int UG = U;
     constexpr int U0 = 4;   // the generic channel layout holds up to four radial factors per point
     constexpr int UC = 8;   // planes of a column chunk in the scatter and update
