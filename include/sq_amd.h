@@ -77,6 +77,12 @@ typedef struct {
                                 the autotuner at creation */
     float tune_ms[2][2];     /* autotuner probe times per precision for 512 and 1024 threads
                                 (0 where not timed) */
+    int batch[2];            /* nodes per Lanczos workgroup per precision: 1 (one-node kernel) or
+                                2, 4 (batched kernel, nodes stacked in z) */
+    int gather_trip[2];      /* batched kernel's gather trip per precision (0: one-node kernel) */
+    float tune_candidates_ms[2][6]; /* autotuner probe times per precision: one-node 512 / 1024
+                                threads, B=2 trips a / b, B=4 trips a / b (0: not timed) */
+    int union_images[2];     /* most images meeting one union cube, B = 2 and B = 4 */
 } sq_info;
 
 /* Message of the last failing call on this thread ("" if none). */

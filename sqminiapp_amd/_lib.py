@@ -81,6 +81,8 @@ class Info(C.Structure):
         ("rcut", C.c_double), ("coeffs", (C.c_double * 11) * 3), ("gamma", (C.c_double * 4) * 8),
         ("cutoff", (C.c_double * 3) * 8), ("chi_entries", C.c_longlong), ("kernel", C.c_int), ("kmax", C.c_int),
         ("workgroup", C.c_int * 2), ("tune_ms", (C.c_float * 2) * 2),
+        ("batch", C.c_int * 2), ("gather_trip", C.c_int * 2), ("tune_candidates_ms", (C.c_float * 6) * 2),
+        ("union_images", C.c_int * 2),
     ]
 
 

@@ -55,6 +55,22 @@ struct sq_ctx {
     float tune_ms[2][2] = {};     // autotuner probe time per precision and shape (0: 512, 1: 1024)
     size_t zlds = 0;
     int z_tlo = 0, z_tdim = 0, z_toff = 0, z_tcen = 0, z_A = 0, z_alpha = 0, z_red = 0, z_plane = 0;
+    // batched kernel (sq_zbatch.hip): B = 2 or 4 nodes stacked in z per workgroup.
+    // Per B: images per union cube, LDS size and offsets (tlo, tdim, toff, tcen,
+    // A, alpha, red, local, plane), resident grid.
+    bool zbatch = false;
+    int zb_cap[5] = {}, zb_grid[2][5] = {};   // grid per precision and B
+    size_t zb_lds[5] = {};
+    int zb_off[5][9] = {};
+    // the configuration the autotuner chose per precision: nodes per workgroup
+    // (1: the one-node z-march kernel in its zwg shape) and gather trip
+    int cfg_B[2] = {1, 1}, cfg_ug[2] = {0, 0};
+    float tune_cand[2][6] = {};   // probe ms: one-node 512 / 1024 threads, B=2 trips a/b, B=4 trips a/b
+    // batch list of the last node range
+    int2 *batches = nullptr;
+    long batch_cap = 0, n_batches = 0, bat_begin = -1, bat_count = -1;
+    int bat_B = 0;
+    int *probe_ids = nullptr;     // the autotuner's node set (one-node kernel form)
     hipStream_t stream = nullptr;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     // device tables
@@ -126,7 +142,7 @@ void free_ctx(sq_ctx *c)
     void *ptrs[] = {c->img_dimp, c->rad64, c->rad32, c->img_rad, c->img_cen, c->chi64, c->chi32, c->img_lo, c->img_dim, c->img_off, c->img_atom, c->img_type, c->gamma,
                     c->cov_off, c->cov_img, c->diag64, c->diag32, c->c64, c->c32, c->scratch, c->a, c->b,
                     c->lam, c->w, c->lmin, c->lmax, c->qlws, c->status, c->partial, c->partial2, c->out,
-                    c->queue, c->nsteps};
+                    c->queue, c->nsteps, c->batches, c->probe_ids};
     void *scf_ptrs[] = {c->scf.rho, c->scf.rho_out, c->scf.b, c->scf.phi, c->scf.eps, c->scf.vxc, c->scf.veff, c->scf.r,
                         c->scf.p, c->scf.Ap, c->scf.f, c->scf.x_prev, c->scf.f_prev, c->scf.dX, c->scf.dF, c->scf.g,
                         c->scf.s, c->scf.part, c->scf.ku, c->scf.ky};
@@ -178,6 +194,69 @@ int max_images_per_cube(const System &S)
         for (int j = 0; j < ly; j++)
             for (int i = 0; i < lx; i++) mx = std::max(mx, at(i, j, k));
     return mx;
+}
+
+// Largest number of images any union cube of a batch of B nodes stacked in z
+// (lowest node k: planes k - hw .. k + hw + B - 1) meets.
+int max_images_per_union(const System &S, int B)
+{
+    const int lx = S.box[3], ly = S.box[4], lz = S.box[5];
+    std::vector<int> diff((size_t)(lx + 1) * (ly + 1) * (lz + 1), 0);
+    auto at = [&](int i, int j, int k) -> int & { return diff[((size_t)k * (ly + 1) + j) * (lx + 1) + i]; };
+    for (const Image &im : S.images) {
+        int lo[3], hi[3];
+        bool empty = false;
+        for (int d = 0; d < 3; d++) {
+            const int down = d == 2 ? S.hw[d] + B - 1 : S.hw[d];
+            lo[d] = std::max(im.lo[d] - down, S.box[d]) - S.box[d];
+            hi[d] = std::min(im.hi[d] + S.hw[d], S.box[d] + S.box[3 + d] - 1) - S.box[d];
+            if (hi[d] < lo[d]) empty = true;
+        }
+        if (empty) continue;
+        for (int s = 0; s < 8; s++) {
+            int i = (s & 1) ? hi[0] + 1 : lo[0], j = (s & 2) ? hi[1] + 1 : lo[1], k = (s & 4) ? hi[2] + 1 : lo[2];
+            int sign = ((s & 1) ? -1 : 1) * ((s & 2) ? -1 : 1) * ((s & 4) ? -1 : 1);
+            at(i, j, k) += sign;
+        }
+    }
+    for (int k = 0; k <= lz; k++)
+        for (int j = 0; j <= ly; j++)
+            for (int i = 1; i <= lx; i++) at(i, j, k) += at(i - 1, j, k);
+    for (int k = 0; k <= lz; k++)
+        for (int j = 1; j <= ly; j++)
+            for (int i = 0; i <= lx; i++) at(i, j, k) += at(i, j - 1, k);
+    for (int k = 1; k <= lz; k++)
+        for (int j = 0; j <= ly; j++)
+            for (int i = 0; i <= lx; i++) at(i, j, k) += at(i, j, k - 1);
+    int mx = 0;
+    for (int k = 0; k < lz; k++)
+        for (int j = 0; j < ly; j++)
+            for (int i = 0; i < lx; i++) mx = std::max(mx, at(i, j, k));
+    return mx;
+}
+
+// Batches of up to B nodes (i, j, k .. k+nb-1) covering the rank-box ids
+// [begin, begin + count): for fixed (i, j) the ids of a range are a run of
+// consecutive k, so each batch is {lowest id, nb}.  Blocks of B planes from
+// the range's first plane, (i, j) fastest within a block.
+void build_batches(const System &S, long begin, long count, int B, std::vector<int2> &out)
+{
+    out.clear();
+    const long plane = (long)S.box[3] * S.box[4], end = begin + count;
+    const long k_lo = begin / plane, k_hi = (end - 1) / plane;
+    for (long kb = k_lo; kb <= k_hi; kb += B)
+        for (long ij = 0; ij < plane; ij++) {
+            long first = -1;
+            int nb = 0;
+            for (long k = kb; k < std::min(kb + B, k_hi + 1); k++) {
+                const long id = k * plane + ij;
+                if (id >= begin && id < end) {
+                    if (first < 0) first = id;
+                    nb++;
+                }
+            }
+            if (nb) out.push_back(make_int2((int)first, nb));
+        }
 }
 
 // Largest number of images any (x,y) column of any cube meets: per image, the
@@ -368,6 +447,29 @@ int create_device(sq_ctx *c, int device)
         c->zlds = z;
         if (c->zlds > 150 * 1024) c->zmarch = false;
     }
+    // batched kernel: the one-node kernel's conditions, s+d channels only
+    bool sd = true;
+    for (int t = 0; t < S.ntype; t++) {
+        const Pseudopotential &ps = S.psd[t];
+        if (!(ps.nlm() == 6 && ps.ch_l[0] == 0 && ps.ch_l[1] == 2 && ps.ch_l[5] == 2)) sd = false;
+    }
+    c->zbatch = c->zmarch && sd && getenv("SQ_NO_BATCH") == nullptr;
+    for (int B = 2; B <= 4 && c->zbatch; B += 2) {
+        const int cap = std::max(1, max_images_per_union(S, B));
+        size_t z = align16((size_t)cap * sizeof(int));
+        int *off = c->zb_off[B];
+        off[0] = (int)z; z = align16(z + (size_t)cap * sizeof(int4));      // tlo
+        off[1] = (int)z; z = align16(z + (size_t)cap * sizeof(int4));      // tdim
+        off[2] = (int)z; z = align16(z + (size_t)cap * sizeof(int64_t));   // toff
+        off[3] = (int)z; z = align16(z + (size_t)cap * sizeof(double4));   // tcen
+        off[4] = (int)z; z = align16(z + (size_t)cap * B * 6 * sizeof(double));   // A
+        off[5] = (int)z; z = align16(z + (size_t)cap * B * 6 * sizeof(double));   // alpha
+        off[6] = (int)z; z = align16(z + (size_t)(B + 1) * 16 * sizeof(double) + 64);   // red
+        off[7] = (int)z; z = align16(z + (size_t)B * 2 * (S.nqp + 1) * sizeof(double));   // early-exit rule
+        off[8] = (int)z; z = align16(z + 2 * (size_t)(S.nloc[0] + 12) * (S.nloc[1] + 12) * 16);   // plane (16 B a point)
+        c->zb_cap[B] = cap;
+        c->zb_lds[B] = cap <= 256 && z <= 150 * 1024 ? z : 0;
+    }
     return SQ_OK;
 }
 
@@ -474,12 +576,29 @@ KParams make_params(const sq_ctx *c, int mode, int precision, long begin, long c
     return P;
 }
 
-// Grids of the z-march kernel (resident workgroups, a multiple of 8 so the
-// XCD split is even), the Lanczos scratch sized for the largest grid, and the
-// workgroup shape per precision: both shapes (512 threads with two columns a
-// thread, 1024 with one) are timed on a short probe -- the first grid's worth
-// of nodes, six Lanczos steps -- and the faster one is kept.  SQ_ZMARCH_WG=512
-// or 1024 fixes the shape instead.
+// Parameters of the batched kernel for B nodes a workgroup over a batch list.
+KParams make_batch_params(const sq_ctx *c, int mode, int precision, int B, const int2 *batch, long n_batch)
+{
+    KParams P = make_params(c, mode, precision, 0, n_batch, true);
+    const int *off = c->zb_off[B];
+    P.lds_tlo = off[0];
+    P.lds_tdim = off[1];
+    P.lds_toff = off[2];
+    P.lds_tcen = off[3];
+    P.lds_A = off[4];
+    P.lds_alpha = off[5];
+    P.lds_red = off[6];
+    P.lds_local = off[7];
+    P.lds_plane = off[8];
+    P.img_cap = c->zb_cap[B];
+    P.batch = batch;
+    P.node_count = n_batch;
+    P.node_begin = 0;
+    return P;
+}
+
+// Resident grid of the one-node z-march kernel for a workgroup shape (a
+// multiple of 8 so the XCD split is even); 0 if it cannot launch.
 static int zmarch_grid(sq_ctx *c, int prec, int wg)
 {
     KParams P = make_params(c, SQ_NL_KB, prec, 0, 1, true);
@@ -488,6 +607,87 @@ static int zmarch_grid(sq_ctx *c, int prec, int wg)
     return occ <= 0 ? 0 : std::max(8, (c->n_cu * occ) & ~7);
 }
 
+// The gather trips the batched kernel is built with, per precision: a, b.
+static int batch_trip(int prec, int which) { return prec == 0 ? 1 + which : 4 + 4 * which; }
+
+static int zbatch_grid(sq_ctx *c, int prec, int B, int ug)
+{
+    if (!c->zb_lds[B] || !zbatch_supported(prec, B, ug)) return 0;
+    KParams P = make_batch_params(c, SQ_NL_KB, prec, B, nullptr, 1);
+    const int occ = zbatch_occupancy(P, prec, B, ug, c->zb_lds[B]);
+    return occ <= 0 ? 0 : std::max(8, (c->n_cu * occ) & ~7);
+}
+
+// Uploads a batch list into the context's buffer (grown as needed).
+static int upload_batches(sq_ctx *c, const std::vector<int2> &v)
+{
+    if ((long)v.size() > c->batch_cap) {
+        if (c->batches) HIP_TRY(hipFree(c->batches));
+        c->batches = nullptr;
+        c->batch_cap = std::max<long>((long)v.size(), 1024);
+        HIP_TRY(hipMalloc((void **)&c->batches, c->batch_cap * sizeof(int2)));
+    }
+    if (!v.empty()) HIP_TRY(hipMemcpy(c->batches, v.data(), v.size() * sizeof(int2), hipMemcpyHostToDevice));
+    c->n_batches = (long)v.size();
+    return SQ_OK;
+}
+
+// The batch list of rank-box ids [begin, begin + count) for B, cached.
+static int ensure_batches(sq_ctx *c, int B, long begin, long count)
+{
+    if (c->bat_B == B && c->bat_begin == begin && c->bat_count == count) return SQ_OK;
+    std::vector<int2> v;
+    build_batches(c->sys, begin, count, B, v);
+    int rc = upload_batches(c, v);
+    if (rc) return rc;
+    c->bat_B = B;
+    c->bat_begin = begin;
+    c->bat_count = count;
+    return SQ_OK;
+}
+
+// One Lanczos launch of configuration (B, shape or trip) on the probe's node
+// set, timed by HIP events on the context's stream.
+static int probe_launch(sq_ctx *c, int prec, int B, int shape, int nqp, long nprobe, float *ms)
+{
+    hipError_t e = hipSuccess;
+    for (int rep = 0; rep < 2; rep++) {   // the first launch loads the code object
+        HIP_TRY(hipMemsetAsync(c->queue, 0, 8 * sizeof(int), c->stream));
+        HIP_TRY(hipEventRecord(c->ev[0], c->stream));
+        if (B == 1) {
+            KParams P = make_params(c, SQ_NL_KB, prec, 0, nprobe, true);
+            P.zwg = shape;
+            P.node_list = c->probe_ids;
+            P.nqp = nqp;
+            P.exit_tol = 0;
+            const int grid = (int)std::min<long>(zmarch_grid(c, prec, shape), std::max<long>(8, (nprobe + 7) & ~7L));
+            e = launch_zmarch(P, prec, grid, c->zlds, c->stream);
+        } else {
+            KParams P = make_batch_params(c, SQ_NL_KB, prec, B, c->batches, c->n_batches);
+            P.nqp = nqp;
+            P.exit_tol = 0;
+            const int grid = (int)std::min<long>(c->zb_grid[prec][B], std::max<long>(8, (c->n_batches + 7) & ~7L));
+            e = launch_zbatch(P, prec, B, shape, grid, c->zb_lds[B], c->stream);
+        }
+        if (e != hipSuccess) return set_err(SQ_ERR_DEVICE, std::string("autotuner probe: ") + hipGetErrorString(e));
+        HIP_TRY(hipEventRecord(c->ev[1], c->stream));
+    }
+    HIP_TRY(hipEventSynchronize(c->ev[1]));
+    HIP_TRY(hipEventElapsedTime(ms, c->ev[0], c->ev[1]));
+    return SQ_OK;
+}
+
+// Grids, the Lanczos scratch sized for the largest configuration, and the
+// autotuner.  Per precision it times every configuration on one node set --
+// four planes of the rank box's middle (fewer if the box is thinner), the
+// first min(plane, 1024) nodes of each, 20 Lanczos steps (fewer if the deck
+// runs fewer) -- and keeps the fastest:
+//   the one-node z-march kernel with 512 threads (two columns a thread) or
+//   1024 (one); the batched kernel with B = 2 (FP64, FP32) or 4 (FP32) nodes
+//   a workgroup, each with two gather trips.
+// The same nodes and steps for every candidate, so the times compare
+// directly.  SQ_ZMARCH_WG=512|1024 fixes the one-node kernel in that shape;
+// SQ_BATCH=1|2|4 (and SQ_BATCH_TRIP) fixes the batch size (and trip).
 int finish_device_setup(sq_ctx *c)
 {
     const System &S = c->sys;
@@ -501,37 +701,89 @@ int finish_device_setup(sq_ctx *c)
                 maxgrid = std::max(maxgrid, grids[prec][sh]);
             }
     }
-    HIP_TRY(hipMalloc(&c->scratch, (size_t)maxgrid * 3 * S.nc() * sizeof(double)));
-    if (!c->zmarch) return SQ_OK;
-    const char *fixed = getenv("SQ_ZMARCH_WG");
-    for (int prec = 0; prec < 2; prec++) {
-        int best = 0;
-        if (fixed) best = atoi(fixed) == 1024 && grids[prec][1] > 0 ? 1 : 0;
-        else if (grids[prec][1] > 0) {
-            // each probe runs one node per workgroup of its shape's full grid; the
-            // shapes' grids differ when their occupancies do, so compare time per node
-            double per_node[2] = {0.0, 0.0};
-            for (int sh = 0; sh < 2; sh++) {
-                KParams P = make_params(c, SQ_NL_KB, prec, 0, std::min<long>(grids[prec][sh], S.n_nodes()), true);
-                P.zwg = sh ? 1024 : 512;
-                P.nqp = std::min(P.nqp, 5);
-                P.exit_tol = 0;
-                const int grid = (int)std::min<long>(grids[prec][sh], std::max<long>(8, (P.node_count + 7) & ~7L));
-                for (int rep = 0; rep < 2; rep++) {   // the first launch loads the code object
-                    HIP_TRY(hipMemsetAsync(c->queue, 0, 8 * sizeof(int), c->stream));
-                    HIP_TRY(hipEventRecord(c->ev[0], c->stream));
-                    HIP_TRY(launch_zmarch(P, prec, grid, c->zlds, c->stream));
-                    HIP_TRY(hipEventRecord(c->ev[1], c->stream));
+    size_t scratch = (size_t)maxgrid * 3 * S.nc() * sizeof(double);
+    if (c->zmarch && c->zbatch) {
+        const size_t ncol = (size_t)S.nloc[0] * S.nloc[1];
+        for (int B = 2; B <= 4; B += 2)
+            for (int prec = 0; prec < 2; prec++)
+                for (int w = 0; w < 2; w++) {
+                    const int g = zbatch_grid(c, prec, B, batch_trip(prec, w));
+                    if (!g) continue;
+                    c->zb_grid[prec][B] = std::max(c->zb_grid[prec][B], g);
+                    const size_t el = prec == 0 ? sizeof(double) : sizeof(float);
+                    scratch = std::max(scratch, (size_t)g * 3 * (S.nloc[2] + B - 1) * ncol * B * el);
                 }
-                HIP_TRY(hipEventSynchronize(c->ev[1]));
-                HIP_TRY(hipEventElapsedTime(&c->tune_ms[prec][sh], c->ev[0], c->ev[1]));
-                per_node[sh] = c->tune_ms[prec][sh] / (double)P.node_count;
-            }
-            best = per_node[1] < per_node[0] ? 1 : 0;
-        }
-        c->zwg[prec] = best ? 1024 : 512;
-        c->zgrid[prec] = grids[prec][best];
     }
+    HIP_TRY(hipMalloc(&c->scratch, scratch));
+    if (!c->zmarch) return SQ_OK;
+    for (int prec = 0; prec < 2; prec++) {
+        c->zwg[prec] = 512;
+        c->zgrid[prec] = grids[prec][0];
+        c->cfg_B[prec] = 1;
+        c->cfg_ug[prec] = 0;
+    }
+    const char *fixed_wg = getenv("SQ_ZMARCH_WG");
+    const char *fixed_b = getenv("SQ_BATCH");
+    const char *fixed_t = getenv("SQ_BATCH_TRIP");
+    // the probe's node set
+    const int lz = S.box[5];
+    const long plane = (long)S.box[3] * S.box[4];
+    const int np = std::min(4, lz), k0 = std::max(0, lz / 2 - 2);
+    const long M = std::min<long>(plane, 1024);
+    std::vector<int> ids;
+    std::vector<int2> bl[5];
+    for (long ij = 0; ij < M; ij++)
+        for (int k = k0; k < k0 + np; k++) ids.push_back((int)((long)k * plane + ij));
+    for (int B = 2; B <= 4; B += 2)
+        for (long ij = 0; ij < M; ij++)
+            for (int kb = k0; kb < k0 + np; kb += B)
+                bl[B].push_back(make_int2((int)((long)kb * plane + ij), std::min(B, k0 + np - kb)));
+    int rc;
+    if ((rc = dev_upload(&c->probe_ids, ids.data(), ids.size()))) return rc;
+    const int nqp = std::min(S.nqp, 20);
+    for (int prec = 0; prec < 2; prec++) {
+        // candidates: 0/1 one-node 512/1024 threads, 2/3 B=2 trips a/b, 4/5 B=4 trips a/b
+        bool allow[6];
+        for (int k = 0; k < 6; k++) {
+            const int B = k < 2 ? 1 : (k < 4 ? 2 : 4);
+            allow[k] = B == 1 ? grids[prec][k] > 0
+                              : (c->zbatch && zbatch_grid(c, prec, B, batch_trip(prec, k & 1)) > 0);
+            if (fixed_wg) allow[k] = B == 1 && (atoi(fixed_wg) == 1024) == (k == 1) && grids[prec][k] > 0;
+            if (fixed_b && atoi(fixed_b) != B) allow[k] = false;
+            if (fixed_t && B > 1 && atoi(fixed_t) != batch_trip(prec, k & 1)) allow[k] = false;
+        }
+        int n_allowed = 0, best = -1;
+        for (int k = 0; k < 6; k++) n_allowed += allow[k];
+        if (n_allowed == 0) allow[0] = grids[prec][0] > 0, n_allowed = 1;
+        float best_ms = 0.f;
+        for (int k = 0; k < 6; k++) {
+            if (!allow[k]) continue;
+            if (n_allowed == 1) { best = k; break; }
+            const int B = k < 2 ? 1 : (k < 4 ? 2 : 4);
+            if (B > 1 && (rc = upload_batches(c, bl[B]))) return rc;
+            float ms = 0.f;
+            if ((rc = probe_launch(c, prec, B, B == 1 ? (k ? 1024 : 512) : batch_trip(prec, k & 1), nqp, (long)ids.size(),
+                                   &ms)))
+                return rc;
+            c->tune_cand[prec][k] = ms;
+            if (k < 2) c->tune_ms[prec][k] = ms;
+            if (best < 0 || ms < best_ms) { best = k; best_ms = ms; }
+        }
+        if (best < 0) best = 0;
+        if (best < 2) {
+            c->zwg[prec] = best ? 1024 : 512;
+            c->zgrid[prec] = grids[prec][best];
+        } else {
+            const int B = best < 4 ? 2 : 4;
+            c->cfg_B[prec] = B;
+            c->cfg_ug[prec] = batch_trip(prec, best & 1);
+            // the one-node kernel stays available (force pass, node lists) in the 1024 shape when it fits
+            c->zwg[prec] = grids[prec][1] > 0 ? 1024 : 512;
+            c->zgrid[prec] = grids[prec][grids[prec][1] > 0 ? 1 : 0];
+        }
+    }
+    c->bat_B = 0;   // the probe's list is not a node range
+    c->bat_begin = c->bat_count = -1;
     return SQ_OK;
 }
 
@@ -643,10 +895,15 @@ int sq_get_info(sq_ctx *c, sq_info *I)
     I->kernel = c->zmarch ? 1 : 0;
     I->kmax = c->kmax;
     for (int prec = 0; prec < 2; prec++) {
-        I->workgroup[prec] = c->zmarch ? c->zwg[prec] : 1024;
+        I->workgroup[prec] = c->zmarch ? (c->cfg_B[prec] > 1 ? 1024 : c->zwg[prec]) : 1024;
         I->tune_ms[prec][0] = c->tune_ms[prec][0];
         I->tune_ms[prec][1] = c->tune_ms[prec][1];
+        I->batch[prec] = c->zmarch ? c->cfg_B[prec] : 1;
+        I->gather_trip[prec] = c->cfg_ug[prec];
+        for (int k = 0; k < 6; k++) I->tune_candidates_ms[prec][k] = c->tune_cand[prec][k];
     }
+    I->union_images[0] = c->zb_cap[2];
+    I->union_images[1] = c->zb_cap[4];
     return SQ_OK;
 }
 
@@ -670,7 +927,17 @@ int sq_run(sq_ctx *c, int mode, int precision, long begin, long count)
     c->have_run = false;
     HIP_TRY(hipEventRecord(c->ev[0], c->stream));
     c->last_exit = false;
-    if (c->zmarch) {
+    if (c->zmarch && c->cfg_B[precision] > 1) {
+        const int B = c->cfg_B[precision];
+        int rc = ensure_batches(c, B, begin, count);
+        if (rc) return rc;
+        KParams P = make_batch_params(c, mode, precision, B, c->batches, c->n_batches);
+        int grid = c->zb_grid[precision][B];
+        if (c->n_batches < grid) grid = std::max<long>(8, (c->n_batches + 7) & ~7L);
+        c->last_exit = P.exit_tol > 0;
+        HIP_TRY(hipMemsetAsync(c->queue, 0, 8 * sizeof(int), c->stream));
+        HIP_TRY(launch_zbatch(P, precision, B, c->cfg_ug[precision], grid, c->zb_lds[B], c->stream));
+    } else if (c->zmarch) {
         KParams P = make_params(c, mode, precision, begin, count, true);
         int grid = c->zgrid[precision];
         if (count < grid) grid = std::max<long>(8, (count + 7) & ~7L);

@@ -64,11 +64,18 @@ struct KParams {
     double *rec;                 // [node_count][nqp][rec_slots][nlm_max]
     int *rec_img;                // [node_count][rec_slots] image id, -1 past the node's count
     int rec_slots;
+    // batched kernel: node_count batches of up to B nodes (i, j, k .. k+nb-1),
+    // each {id of its lowest node, nb}
+    const int2 *batch;
 };
 
 hipError_t launch_lanczos(const KParams &P, int precision, int grid, size_t lds, hipStream_t st);
 hipError_t launch_zmarch(const KParams &P, int precision, int grid, size_t lds, hipStream_t st);
 int zmarch_occupancy(const KParams &P, int precision, size_t lds);
+// batched kernel (sq_zbatch.hip): B nodes stacked in z per 1024-thread workgroup
+bool zbatch_supported(int precision, int B, int ug);
+hipError_t launch_zbatch(const KParams &P, int precision, int B, int ug, int grid, size_t lds, hipStream_t st);
+int zbatch_occupancy(const KParams &P, int precision, int B, int ug, size_t lds);
 hipError_t launch_apply(const KParams &P, long id, const double *x, double *y, int nonlocal_only, size_t lds,
                         hipStream_t st);
 hipError_t launch_ql(const double *a, const double *b, int n, int stride, long count, double *lam, double *w,
