@@ -61,7 +61,7 @@ struct sq_ctx {
     bool zbatch = false;
     int zb_cap[5] = {}, zb_grid[2][5] = {};   // grid per precision and B
     size_t zb_lds[5] = {};
-    int zb_off[5][9] = {};
+    int zb_off[5][10] = {};
     // the configuration the autotuner chose per precision: nodes per workgroup
     // (1: the one-node z-march kernel in its zwg shape) and gather trip
     int cfg_B[2] = {1, 1}, cfg_ug[2] = {0, 0};
@@ -71,6 +71,8 @@ struct sq_ctx {
     long batch_cap = 0, n_batches = 0, bat_begin = -1, bat_count = -1;
     int bat_B = 0;
     int *probe_ids = nullptr;     // the autotuner's node set (one-node kernel form)
+    double *bas64 = nullptr;      // column basis tables of the batched kernel
+    float *bas32 = nullptr;
     hipStream_t stream = nullptr;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     // device tables
@@ -142,7 +144,7 @@ void free_ctx(sq_ctx *c)
     void *ptrs[] = {c->img_dimp, c->rad64, c->rad32, c->img_rad, c->img_cen, c->chi64, c->chi32, c->img_lo, c->img_dim, c->img_off, c->img_atom, c->img_type, c->gamma,
                     c->cov_off, c->cov_img, c->diag64, c->diag32, c->c64, c->c32, c->scratch, c->a, c->b,
                     c->lam, c->w, c->lmin, c->lmax, c->qlws, c->status, c->partial, c->partial2, c->out,
-                    c->queue, c->nsteps, c->batches, c->probe_ids};
+                    c->queue, c->nsteps, c->batches, c->probe_ids, c->bas64, c->bas32};
     void *scf_ptrs[] = {c->scf.rho, c->scf.rho_out, c->scf.b, c->scf.phi, c->scf.eps, c->scf.vxc, c->scf.veff, c->scf.r,
                         c->scf.p, c->scf.Ap, c->scf.f, c->scf.x_prev, c->scf.f_prev, c->scf.dX, c->scf.dF, c->scf.g,
                         c->scf.s, c->scf.part, c->scf.ku, c->scf.ky};
@@ -467,8 +469,18 @@ int create_device(sq_ctx *c, int device)
         off[6] = (int)z; z = align16(z + (size_t)(B + 1) * 16 * sizeof(double) + 64);   // red
         off[7] = (int)z; z = align16(z + (size_t)B * 2 * (S.nqp + 1) * sizeof(double));   // early-exit rule
         off[8] = (int)z; z = align16(z + 2 * (size_t)(S.nloc[0] + 12) * (S.nloc[1] + 12) * 16);   // plane (16 B a point)
+        off[9] = (int)z; z = align16(z + (size_t)cap * sizeof(int4));      // atom runs
         c->zb_cap[B] = cap;
         c->zb_lds[B] = cap <= 256 && z <= 150 * 1024 ? z : 0;
+    }
+    if (c->zbatch) {
+        // (t0, t2, t2 z, t2 z^2) per image box point, from the radial factors
+        const int64_t n4 = std::max<int64_t>(1, 2 * S.rad_total);
+        HIP_TRY(hipMalloc((void **)&c->bas64, n4 * sizeof(double)));
+        HIP_TRY(hipMalloc((void **)&c->bas32, n4 * sizeof(float)));
+        HIP_TRY(launch_basis_tables(c->rad64, M, c->img_lo, c->img_dim, c->img_rad, c->img_cen, S.h[2], S.range[2],
+                                    c->bas64, c->bas32, c->stream));
+        HIP_TRY(hipStreamSynchronize(c->stream));
     }
     return SQ_OK;
 }
@@ -590,6 +602,8 @@ KParams make_batch_params(const sq_ctx *c, int mode, int precision, int B, const
     P.lds_red = off[6];
     P.lds_local = off[7];
     P.lds_plane = off[8];
+    P.lds_run = off[9];
+    P.basis = precision == SQ_FP64 ? (const void *)c->bas64 : (const void *)c->bas32;
     P.img_cap = c->zb_cap[B];
     P.batch = batch;
     P.node_count = n_batch;

@@ -67,6 +67,8 @@ struct KParams {
     // batched kernel: node_count batches of up to B nodes (i, j, k .. k+nb-1),
     // each {id of its lowest node, nb}
     const int2 *batch;
+    const void *basis;           // (t0, t2, t2 z, t2 z^2) per image box point (element type = precision)
+    int lds_run;                 // batched kernel: per-image atom run and type
 };
 
 hipError_t launch_lanczos(const KParams &P, int precision, int grid, size_t lds, hipStream_t st);
@@ -76,6 +78,8 @@ int zmarch_occupancy(const KParams &P, int precision, size_t lds);
 bool zbatch_supported(int precision, int B, int ug);
 hipError_t launch_zbatch(const KParams &P, int precision, int B, int ug, int grid, size_t lds, hipStream_t st);
 int zbatch_occupancy(const KParams &P, int precision, int B, int ug, size_t lds);
+hipError_t launch_basis_tables(const double *rad, int n_images, const int4 *lo, const int4 *dim, const int64_t *roff,
+                               const double4 *cen, double hz, double rz, double *b64, float *b32, hipStream_t st);
 hipError_t launch_apply(const KParams &P, long id, const double *x, double *y, int nonlocal_only, size_t lds,
                         hipStream_t st);
 hipError_t launch_ql(const double *a, const double *b, int n, int stride, long count, double *lam, double *w,

@@ -114,7 +114,7 @@ __global__ void __launch_bounds__(1024) lanczos_zbatch(KParams P)
 {
     constexpr int WG = 1024, NW = WG / 64, O = 6, W = 2 * O + 1, NLM = 6;
     using VB = typename VecOf<T, B>::type;
-    using V2 = typename std::conditional<sizeof(T) == 8, double2, float2>::type;
+    typedef T V4 __attribute__((ext_vector_type(4)));
     extern __shared__ __align__(16) unsigned char smem[];
     int *img_list = reinterpret_cast<int *>(smem);
     int4 *tlo = reinterpret_cast<int4 *>(smem + P.lds_tlo);
@@ -126,6 +126,7 @@ __global__ void __launch_bounds__(1024) lanczos_zbatch(KParams P)
     double *red = reinterpret_cast<double *>(smem + P.lds_red);
     double *s_ab = reinterpret_cast<double *>(smem + P.lds_local);   // [B][2][nqp+1] (early exit)
     VB *plane = reinterpret_cast<VB *>(smem + P.lds_plane);
+    int4 *trun = reinterpret_cast<int4 *>(smem + P.lds_run);          // [img]: first, last image of its atom; type
     __shared__ int s_count;
     __shared__ T s_w[3][8];
     __shared__ long s_next;
@@ -146,14 +147,13 @@ __global__ void __launch_bounds__(1024) lanczos_zbatch(KParams P)
     const bool has = col < ncol;
     const int cj = has ? col / nx : 0, ci = has ? col - cj * nx : 0;
     const int pcol = (cj + O) * px + ci + O;
-    const T hzt = (T)P.hgrid[2];
     for (int q = tid; q < 2 * pplane; q += WG) plane[q] = VB{};   // the halo stays zero
     const size_t vlen = (size_t)nzs * ncol * B;
     T *__restrict__ buf0 = reinterpret_cast<T *>(P.scratch) + (size_t)blockIdx.x * 3 * vlen;
     T *__restrict__ buf1 = buf0 + vlen;
     T *__restrict__ Bu = buf1 + vlen;
     const T *__restrict__ diag_ext = reinterpret_cast<const T *>(P.diag_ext);
-    const T *__restrict__ rad = reinterpret_cast<const T *>(P.rad);
+    const T *__restrict__ bas = reinterpret_cast<const T *>(P.basis);   // (t0, t2, t2 z, t2 z^2) per image point
     if (tid < 21) {
         if constexpr (sizeof(T) == 8) s_w[tid / 7][tid % 7] = P.cw64[tid / 7][tid % 7];
         else s_w[tid / 7][tid % 7] = P.cw32[tid / 7][tid % 7];
@@ -234,6 +234,15 @@ __global__ void __launch_bounds__(1024) lanczos_zbatch(KParams P)
             __syncthreads();
             continue;
         }
+        // ---- per image: the run of list entries holding the same atom (the list is
+        // ascending, so an atom's images are contiguous) and its type
+        for (int li = tid; li < nimg; li += WG) {
+            const int atom = P.img_atom[img_list[li]];
+            int s0 = li, e0 = li;
+            while (s0 > 0 && P.img_atom[img_list[s0 - 1]] == atom) s0--;
+            while (e0 + 1 < nimg && P.img_atom[img_list[e0 + 1]] == atom) e0++;
+            trun[li] = make_int4(s0, e0, P.img_type[img_list[li]], 0);
+        }
         // ---- per column: bitmask of the images whose x/y box holds it
         unsigned cmask[MW];
         const int gxc = cube_lo[0] + ci, gyc = cube_lo[1] + cj;
@@ -284,12 +293,11 @@ __global__ void __launch_bounds__(1024) lanczos_zbatch(KParams P)
                 const int nxy = n[0] * n[1];
                 const float inv0 = 1.0f / (float)n[0];
                 const int zs2 = dm.x * dm.y;
-                // z of a plane from the grid plane nearest the image centre, so a
-                // node's values depend neither on where the batch's union starts
-                // nor on how a rank box clips the image box
+                // the basis tables hold z relative to the image centre, so a node's
+                // values depend neither on where the batch's union starts nor on how
+                // a rank box clips the image box; FP64 groups planes by 4 from the
+                // grid plane nearest the centre for the same reason
                 const int pcz = (int)floor((cen.z + P.range[2]) / P.hgrid[2] + 0.5);
-                const T zb = (T)(P.hgrid[2] * pcz - P.range[2] - cen.z);
-                const int dz0 = a0[2] - pcz;
                 const int j = lane & 3;                 // basis index (A) and node index (B, D columns)
                 T acc[NLM];
 #pragma unroll
@@ -305,8 +313,8 @@ __global__ void __launch_bounds__(1024) lanczos_zbatch(KParams P)
                         const int gx = a0[0] + xi, gy = a0[1] + yi;
                         const int c = (gy - cube_lo[1]) * nx + (gx - cube_lo[0]);
                         const T *vp = Bv + ((size_t)(a0[2] - cube_lo[2]) * ncol + c) * B + (j < B ? j : 0);
-                        const T *rp = rad + toff[li] +
-                                      ((int64_t)((a0[2] - lo.z) * dm.y + (gy - lo.y)) * dm.x + (gx - lo.x)) * 2 + (j ? 1 : 0);
+                        const T *rp = bas + 2 * toff[li] +
+                                      ((int64_t)((a0[2] - lo.z) * dm.y + (gy - lo.y)) * dm.x + (gx - lo.x)) * 4 + j;
                         f32x4 D = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll 1
                         for (int zi = 0; zi < n[2]; zi += UG) {
@@ -314,15 +322,11 @@ __global__ void __launch_bounds__(1024) lanczos_zbatch(KParams P)
 #pragma unroll
                             for (int u = 0; u < UG; u++) {
                                 const bool in = ok && zi + u < n[2];
-                                tv[u] = in ? rp[(int64_t)(zi + u) * zs2 * 2] : T(0);
+                                tv[u] = in ? rp[(int64_t)(zi + u) * zs2 * 4] : T(0);
                                 vv[u] = (in && j < B) ? vp[(size_t)(zi + u) * ncol * B] : T(0);
                             }
 #pragma unroll
-                            for (int u = 0; u < UG; u++) {
-                                const T z = fma(hzt, T(dz0 + zi + u), zb);
-                                const T am = j < 2 ? tv[u] : (j == 2 ? tv[u] * z : (tv[u] * z) * z);
-                                D = __builtin_amdgcn_mfma_f32_4x4x1f32(am, vv[u], D, 0, 0, 0);
-                            }
+                            for (int u = 0; u < UG; u++) D = __builtin_amdgcn_mfma_f32_4x4x1f32(tv[u], vv[u], D, 0, 0, 0);
                         }
                         // D[m] = S[basis m][node j] of column q
                         if (ok && j < B) {
@@ -352,8 +356,8 @@ __global__ void __launch_bounds__(1024) lanczos_zbatch(KParams P)
                         const int gx = a0[0] + xi, gy = a0[1] + yi;
                         const int c = (gy - cube_lo[1]) * nx + (gx - cube_lo[0]);
                         const T *vp = Bv + ((int64_t)(zal - cube_lo[2] + k) * ncol + c) * B + (j < B ? j : 0);
-                        const T *rp = rad + toff[li] +
-                                      ((int64_t)((zal + k - lo.z) * dm.y + (gy - lo.y)) * dm.x + (gx - lo.x)) * 2 + (j ? 1 : 0);
+                        const T *rp = bas + 2 * toff[li] +
+                                      ((int64_t)((zal + k - lo.z) * dm.y + (gy - lo.y)) * dm.x + (gx - lo.x)) * 4 + j;
                         double D = 0.0;
 #pragma unroll 1
                         for (int zi = 0; zi < nzal; zi += 4 * UG) {
@@ -362,15 +366,11 @@ __global__ void __launch_bounds__(1024) lanczos_zbatch(KParams P)
                             for (int u = 0; u < UG; u++) {
                                 const int zq = zi + 4 * u + k;
                                 const bool in = ok && zq >= skip && zq < nzal;
-                                tv[u] = in ? rp[(int64_t)(zi + 4 * u) * zs2 * 2] : T(0);
+                                tv[u] = in ? rp[(int64_t)(zi + 4 * u) * zs2 * 4] : T(0);
                                 vv[u] = (in && j < B) ? vp[(int64_t)(zi + 4 * u) * ncol * B] : T(0);
                             }
 #pragma unroll
-                            for (int u = 0; u < UG; u++) {
-                                const T z = fma(hzt, T(zal - pcz + zi + 4 * u + k), zb);
-                                const T am = j < 2 ? tv[u] : (j == 2 ? tv[u] * z : (tv[u] * z) * z);
-                                D = __builtin_amdgcn_mfma_f64_4x4x4f64(am, vv[u], D, 0, 0, 0);
-                            }
+                            for (int u = 0; u < UG; u++) D = __builtin_amdgcn_mfma_f64_4x4x4f64(tv[u], vv[u], D, 0, 0, 0);
                         }
                         // D = S[basis k][node j] of column q (the D row index is lane / 16)
                         if (ok && j < B) {
@@ -402,14 +402,10 @@ __global__ void __launch_bounds__(1024) lanczos_zbatch(KParams P)
             // ---- alpha per node and image from the A of the images of the same atom
             for (int e = tid; e < nimg * B; e += WG) {
                 const int li = e / B, b = e - li * B;
-                const int g = img_list[li];
-                const int atom = P.img_atom[g];
-                const int type = P.img_type[g];
+                const int4 run = trun[li];
+                const int s = run.x, t = run.y;
                 const int nlm = tdim[li].w & 255;
-                int s = li, t = li;
-                while (s > 0 && P.img_atom[img_list[s - 1]] == atom) s--;
-                while (t + 1 < nimg && P.img_atom[img_list[t + 1]] == atom) t++;
-                const double *gam = P.gamma_ch + type * kMaxChannels;
+                const double *gam = P.gamma_ch + run.z * kMaxChannels;
                 if (P.mode == 0) {
                     double al = 0.0;
                     for (int q = s; q <= t; q++)
@@ -575,7 +571,7 @@ __global__ void __launch_bounds__(1024) lanczos_zbatch(KParams P)
                             if (zhi < z0 || zlo > z0 + UC - 1) continue;
                             const double4 cen = tcen[li];
                             const int64_t rb0 = toff[li] +
-                                                ((int64_t)((cube_lo[2] + z0 - lo.z) * dm.y + (gyc - lo.y)) * dm.x + (gxc - lo.x)) * 2;
+                                                ((int64_t)((cube_lo[2] + z0 - lo.z) * dm.y + (gyc - lo.y)) * dm.x + (gxc - lo.x)) * 2;   // 2 rb0 in the basis table
                             const T xc = x0 - (T)cen.x, yc = y0 - (T)cen.y;
                             const T C = T(1.092548430592079), C20 = T(0.315391565252520), C22 = T(0.546274215296040);
                             const T cxy = C * (xc * yc), cy = C * yc, cx = C * xc, qq = -(xc * xc) - yc * yc,
@@ -589,25 +585,19 @@ __global__ void __launch_bounds__(1024) lanczos_zbatch(KParams P)
                                 p1[b] = fma(cy, (T)al[2], cx * (T)al[4]);
                                 p2[b] = T(2) * C20 * (T)al[3];
                             }
-                            const V2 *r2 = reinterpret_cast<const V2 *>(rad + rb0);
-                            const int pcz = (int)floor((cen.z + P.range[2]) / P.hgrid[2] + 0.5);
-                            const T zb = (T)(P.hgrid[2] * pcz - P.range[2] - cen.z);
-                            const int dz0 = cube_lo[2] + z0 - pcz;
-                            V2 t4[UC];
+                            const V4 *r4 = reinterpret_cast<const V4 *>(bas + 2 * rb0);
+                            V4 t4[UC];
 #pragma unroll
                             for (int u = 0; u < UC; u++) {
                                 const bool ok = z0 + u >= zlo && z0 + u <= zhi;
-                                t4[u] = ok ? r2[u * (dm.x * dm.y)] : V2{T(0), T(0)};
+                                t4[u] = ok ? r4[u * (dm.x * dm.y)] : V4{};
                             }
 #pragma unroll
                             for (int u = 0; u < UC; u++) {
-                                const T z = fma(hzt, T(dz0 + u), zb);
                                 if (z0 + u >= zlo && z0 + u <= zhi)
 #pragma unroll
-                                    for (int b = 0; b < B; b++) {
-                                        const T h = fma(z, fma(z, p2[b], p1[b]), p0[b]);
-                                        uu[b][u] = fma(t4[u].y, h, fma(t4[u].x, al0[b], uu[b][u]));
-                                    }
+                                    for (int b = 0; b < B; b++)
+                                        uu[b][u] = fma(t4[u][3], p2[b], fma(t4[u][2], p1[b], fma(t4[u][1], p0[b], fma(t4[u][0], al0[b], uu[b][u]))));
                             }
                         }
                     }
@@ -724,10 +714,47 @@ __global__ void __launch_bounds__(1024) lanczos_zbatch(KParams P)
     }
 }
 
+// Column basis tables of the batched kernel: for every point of an s+d image
+// box, (t0, t2, t2 z, t2 z^2) with t0, t2 the radial factors and z the
+// point's height above the image centre (lsdft_nl.cc:174-190), in FP64 and
+// FP32.  Entry 4 p of image g starts at 2 img_rad[g].
+__global__ void basis_tables(const double *__restrict__ rad, const int4 *__restrict__ lo, const int4 *__restrict__ dim,
+                             const int64_t *__restrict__ roff, const double4 *__restrict__ cen, double hz, double rz,
+                             double *__restrict__ b64, float *__restrict__ b32)
+{
+    const int g = blockIdx.x;
+    const int4 l = lo[g], d = dim[g];
+    const long npt = (long)d.x * d.y * d.z;
+    const int64_t o = roff[g];
+    for (long p = threadIdx.x; p < npt; p += blockDim.x) {
+        const int zi = (int)(p / ((long)d.x * d.y));
+        const double z = hz * (l.z + zi) - rz - cen[g].z;
+        const double t0 = rad[o + 2 * p], t2 = rad[o + 2 * p + 1];
+        const double v[4] = {t0, t2, t2 * z, (t2 * z) * z};
+        for (int k = 0; k < 4; k++) {
+            b64[2 * o + 4 * p + k] = v[k];
+            b32[2 * o + 4 * p + k] = (float)v[k];
+        }
+    }
+}
+
+hipError_t launch_basis_tables(const double *rad, int n_images, const int4 *lo, const int4 *dim, const int64_t *roff,
+                               const double4 *cen, double hz, double rz, double *b64, float *b32, hipStream_t st)
+{
+    if (n_images <= 0) return hipSuccess;
+    hipLaunchKernelGGL(basis_tables, dim3(n_images), dim3(256), 0, st, rad, lo, dim, roff, cen, hz, rz, b64, b32);
+    return hipGetLastError();
+}
+
+// planes of a scatter/update chunk: as many as the registers hold beside the
+// batch's rows (B values a plane) and the basis quadruples
+template <typename T, int B>
+constexpr int scatter_chunk() { return sizeof(T) == 8 || B == 4 ? 4 : 8; }
+
 template <typename T, int B, int UG>
 static hipError_t launch_b(const KParams &P, int grid, size_t lds, hipStream_t st)
 {
-    constexpr int UC = sizeof(T) == 8 ? 4 : 8;
+    constexpr int UC = scatter_chunk<T, B>();
     if (P.img_cap <= 128) hipLaunchKernelGGL((lanczos_zbatch<T, B, 4, UG, UC>), dim3(grid), dim3(1024), lds, st, P);
     else if (P.img_cap <= 256) hipLaunchKernelGGL((lanczos_zbatch<T, B, 8, UG, UC>), dim3(grid), dim3(1024), lds, st, P);
     else return hipErrorInvalidValue;
@@ -737,7 +764,7 @@ static hipError_t launch_b(const KParams &P, int grid, size_t lds, hipStream_t s
 template <typename T, int B, int UG>
 static int occ_b(const KParams &P, size_t lds)
 {
-    constexpr int UC = sizeof(T) == 8 ? 4 : 8;
+    constexpr int UC = scatter_chunk<T, B>();
     int n = 0;
     hipError_t e = P.img_cap <= 128
         ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, lanczos_zbatch<T, B, 4, UG, UC>, 1024, lds)
