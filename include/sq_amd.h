@@ -209,6 +209,8 @@ typedef struct {
     int exit_min, exit_every;   /* its first check and spacing (<= 0: 30 and 5) */
     double kerker;           /* > 0: Kerker-preconditioned mixing with this k0 (1/bohr), the new
                                 density (x - dX g) + beta P (f - dF g), P = q^2/(q^2 + k0^2) */
+    double tol_report;       /* > 0: also record the wall time, iteration and energy at which the
+                                residual first falls below this value (a looser stop of the same run) */
 } sq_scf_options;
 
 typedef struct {
@@ -219,6 +221,10 @@ typedef struct {
        e_el = 1/2 int phi (b - rho); energies of the last input density */
     double wall_s, sq_s, poisson_s;   /* SCF wall time; SQ passes + Fermi level; Poisson solves */
     double mean_rule_size;   /* mean Gauss nodes per node over the SQ passes */
+    int graph_steps;         /* steps replayed from the captured graph of one SCF step (the first
+                                step runs launch by launch; SQ_NO_GRAPHS or early exit: none) */
+    int report_iterations;   /* tol_report: iteration of the first residual below it (0: not reached) */
+    double report_wall_s, report_e_total;   /* wall time and total energy at that iteration */
 } sq_scf_result;
 
 enum sq_field {

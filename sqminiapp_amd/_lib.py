@@ -48,7 +48,8 @@ class ScfOptions(C.Structure):
     _fields_ = [("nonlocal_mode", C.c_int), ("precision", C.c_int), ("max_iter", C.c_int), ("tol", C.c_double),
                 ("mixing", C.c_double), ("history", C.c_int), ("pulay_frequency", C.c_int),
                 ("tol_poisson", C.c_double), ("rho0", C.POINTER(C.c_double)), ("verbose", C.c_int),
-                ("exit_tol", C.c_double), ("exit_min", C.c_int), ("exit_every", C.c_int), ("kerker", C.c_double)]
+                ("exit_tol", C.c_double), ("exit_min", C.c_int), ("exit_every", C.c_int), ("kerker", C.c_double),
+                ("tol_report", C.c_double)]
 
 
 class ScfResult(C.Structure):
@@ -56,7 +57,8 @@ class ScfResult(C.Structure):
                 ("residual", C.c_double), ("lambda_f", C.c_double), ("e_band", C.c_double),
                 ("minus_ts", C.c_double), ("e_xc", C.c_double), ("e_vxc", C.c_double), ("e_el", C.c_double),
                 ("e_self", C.c_double), ("e_total", C.c_double), ("wall_s", C.c_double), ("sq_s", C.c_double),
-                ("poisson_s", C.c_double), ("mean_rule_size", C.c_double)]
+                ("poisson_s", C.c_double), ("mean_rule_size", C.c_double), ("graph_steps", C.c_int),
+                ("report_iterations", C.c_int), ("report_wall_s", C.c_double), ("report_e_total", C.c_double)]
 
 
 class ForceInfo(C.Structure):
@@ -285,10 +287,11 @@ class Context:
 
     def scf(self, mode=SQ_NL_KB, precision=SQ_FP64, max_iter=0, tol=0.0, mixing=0.0, history=0,
             pulay_frequency=0, tol_poisson=0.0, rho0=None, verbose=False, exit_tol=0.0, exit_min=0, exit_every=0,
-            kerker=0.0):
+            kerker=0.0, tol_report=0.0):
         o = ScfOptions()
         lib().sq_default_scf_options(C.byref(o))
         o.kerker = float(kerker)
+        o.tol_report = float(tol_report)
         o.exit_tol, o.exit_min, o.exit_every = float(exit_tol), int(exit_min), int(exit_every)
         o.nonlocal_mode, o.precision, o.max_iter = int(mode), int(precision), int(max_iter)
         o.tol, o.mixing, o.history, o.pulay_frequency = float(tol), float(mixing), int(history), int(pulay_frequency)

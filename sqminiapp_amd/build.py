@@ -18,7 +18,7 @@ EXE = os.path.join(HERE, "sq_main")
 ARCH = os.environ.get("SQ_OFFLOAD_ARCH", "gfx950")
 
 SOURCES = ["sq_setup.cpp", "sq_api.cpp", "sq_lsdft.cpp", "sq_kernels.hip", "sq_zmarch.hip", "sq_scf.hip", "sq_setup.hip",
-           "sq_forces.hip", "sq_mlp.hip", "sq_zbatch.hip"]
+           "sq_forces.hip", "sq_mlp.hip", "sq_zbatch.hip", "sq_scfstep.hip"]
 DRIVER = "sq_main.cpp"
 
 # -ffp-contract=off: products and sums round as written (the FP64 form follows

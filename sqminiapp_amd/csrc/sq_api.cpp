@@ -10,6 +10,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <functional>
 #include <vector>
 
 #include "../../include/sq_amd.h"
@@ -118,6 +119,12 @@ struct sq_ctx {
         // four CG steps of the Poisson solve, captured once (the buffers above are fixed)
         hipGraph_t cg_graph = nullptr;
         hipGraphExec_t cg_exec = nullptr;
+        // the device-side control block of an SCF step, its timing events and the
+        // step captured as one graph (per sq_scf call)
+        ScfCtl *ctl = nullptr;
+        hipEvent_t sev[4] = {nullptr, nullptr, nullptr, nullptr};
+        hipGraph_t step_graph = nullptr;
+        hipGraphExec_t step_exec = nullptr;
         // the last SQ pass of the loop, which sq_forces repeats: its input
         // density's V_eff (left in the diagonal), phi, Fermi level and settings
         bool force_ready = false;
@@ -150,6 +157,11 @@ void free_ctx(sq_ctx *c)
                         c->scf.s, c->scf.part, c->scf.ku, c->scf.ky};
     if (c->scf.cg_exec) (void)hipGraphExecDestroy(c->scf.cg_exec);
     if (c->scf.cg_graph) (void)hipGraphDestroy(c->scf.cg_graph);
+    if (c->scf.step_exec) (void)hipGraphExecDestroy(c->scf.step_exec);
+    if (c->scf.step_graph) (void)hipGraphDestroy(c->scf.step_graph);
+    if (c->scf.ctl) (void)hipFree(c->scf.ctl);
+    for (hipEvent_t e : c->scf.sev)
+        if (e) (void)hipEventDestroy(e);
     for (void *p : scf_ptrs)
         if (p) (void)hipFree(p);
     for (void *p : ptrs)
@@ -1238,6 +1250,11 @@ static int scf_alloc(sq_ctx *c, int hist)
     }
     HIP_TRY(hipMalloc((void **)&F.dX, std::max(1, hist) * ng * sizeof(double)));
     HIP_TRY(hipMalloc((void **)&F.dF, std::max(1, hist) * ng * sizeof(double)));
+    HIP_TRY(hipMemset(F.dX, 0, std::max(1, hist) * ng * sizeof(double)));
+    HIP_TRY(hipMemset(F.dF, 0, std::max(1, hist) * ng * sizeof(double)));
+    if (!F.ctl) HIP_TRY(hipMalloc((void **)&F.ctl, sizeof(ScfCtl)));
+    for (hipEvent_t &e : F.sev)
+        if (!e) HIP_TRY(hipEventCreate(&e));
     HIP_TRY(hipMalloc((void **)&F.g, std::max(1, hist) * sizeof(double)));
     HIP_TRY(hipMalloc((void **)&F.s, 16 * sizeof(double)));
     HIP_TRY(hipMalloc((void **)&F.part, kRedBlocks * sizeof(double)));
@@ -1385,30 +1402,229 @@ static int scf_kerker(sq_ctx *c, const double *u, double *y, double k0, int *ite
     return SQ_OK;
 }
 
-// Gaussian elimination with partial pivoting on the m x m system (row major).
-static bool small_solve(std::vector<double> A, std::vector<double> &x, int m)
+// ------------------------------------------------------------ the SCF step on the device
+// One SCF step as a fixed sequence of launches (sq_scfstep.hip): no host read
+// inside it, so it can be captured as one graph and replayed.  eager: the
+// conjugate-gradient loops are extended from the host until they stop (the
+// first step's cold Poisson solve); otherwise they hold kPoissonBlocks /
+// kKerkerBlocks blocks of four steps whose launches past the stop return at once.
+constexpr int kPoissonBlocks = 192, kKerkerBlocks = 64;
+
+struct StepPlan {
+    int mode, prec;
+    double beta, tol, tolp, kerker;
+    int m, pf;
+    bool eager;
+};
+
+static int cg_blocks(sq_ctx *c, bool eager, int nblocks, int *done, const std::function<int()> &block)
 {
-    for (int col = 0; col < m; col++) {
-        int piv = col;
-        for (int r = col + 1; r < m; r++)
-            if (std::fabs(A[r * m + col]) > std::fabs(A[piv * m + col])) piv = r;
-        if (A[piv * m + col] == 0.0) return false;
-        if (piv != col) {
-            for (int k = 0; k < m; k++) std::swap(A[col * m + k], A[piv * m + k]);
-            std::swap(x[col], x[piv]);
-        }
-        for (int r = col + 1; r < m; r++) {
-            const double f = A[r * m + col] / A[col * m + col];
-            for (int k = col; k < m; k++) A[r * m + k] -= f * A[col * m + k];
-            x[r] -= f * x[col];
-        }
+    if (!eager) {
+        for (int k = 0; k < nblocks; k++)
+            if (int rc = block()) return rc;
+        return SQ_OK;
     }
-    for (int r = m - 1; r >= 0; r--) {
-        double v = x[r];
-        for (int k = r + 1; k < m; k++) v -= A[r * m + k] * x[k];
-        x[r] = v / A[r * m + r];
+    for (;;) {
+        for (int k = 0; k < 16; k++)
+            if (int rc = block()) return rc;
+        int d = 0;
+        HIP_TRY(hipMemcpyAsync(&d, done, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        if (d) return SQ_OK;
     }
-    return true;
+}
+
+// Poisson: phi from f as scf_poisson, the scalars and the stop on the device
+static int enqueue_poisson(sq_ctx *c, const double *f, double tol, bool eager)
+{
+    auto &F = c->scf;
+    const System &S = c->sys;
+    int G[3];
+    if (int rc = cell_dims(c, G)) return rc;
+    const StencilCoeffs sc = stencil_of(S);
+    const long n = F.ng;
+    hipStream_t st = c->stream;
+    ScfCtl *ctl = F.ctl;
+    HIP_TRY(launch_dot(f, nullptr, n, 1.0, F.part, kRedBlocks, F.s + 4, st));
+    HIP_TRY(launch_axpb_mean(2.0 * M_PI, f, F.r, -2.0 * M_PI, F.s + 4, n, nullptr, st));
+    HIP_TRY(launch_dot(F.r, F.r, n, 1.0, F.part, kRedBlocks, F.s + 5, st));
+    HIP_TRY(launch_lap_periodic(F.phi, F.Ap, G, S.order, sc, st));
+    HIP_TRY(launch_axpbypgz(-1.0, F.Ap, 1.0, F.r, 0.0, nullptr, n, st));
+    HIP_TRY(launch_copy(F.p, F.r, n, nullptr, st));
+    HIP_TRY(launch_dot(F.r, F.r, n, 1.0, F.part, kRedBlocks, F.s, st));
+    HIP_TRY(launch_cg_begin(F.s, &ctl->pstop, &ctl->pdone, &ctl->piters, tol * tol, 5, nullptr, st));
+    const int *sk = &ctl->pdone;
+    auto block = [&]() -> int {
+        for (int k = 0; k < 4; k++) {
+            HIP_TRY(launch_lap_periodic(F.p, F.Ap, G, S.order, sc, st, sk));
+            HIP_TRY(launch_dot(F.p, F.Ap, n, 1.0, F.part, kRedBlocks, F.s + 1, st, sk));
+            HIP_TRY(launch_cg_xr(F.phi, F.r, F.p, F.Ap, F.s, n, st, sk));
+            HIP_TRY(launch_dot(F.r, F.r, n, 1.0, F.part, kRedBlocks, F.s + 2, st, sk));
+            HIP_TRY(launch_cg_p(F.p, F.r, F.s, n, st, sk));
+        }
+        HIP_TRY(launch_cg_check(F.s, &ctl->pstop, &ctl->pdone, &ctl->piters, 20000, st));
+        return SQ_OK;
+    };
+    if (int rc = cg_blocks(c, eager, kPoissonBlocks, &ctl->pdone, block)) return rc;
+    HIP_TRY(launch_cg_end(F.s, &ctl->pstop, &ctl->pfail, nullptr, st));
+    HIP_TRY(launch_dot(F.phi, nullptr, n, 1.0, F.part, kRedBlocks, F.s + 4, st));
+    HIP_TRY(launch_axpb_mean(1.0, F.phi, F.phi, -1.0, F.s + 4, n, nullptr, st));
+    return SQ_OK;
+}
+
+// Kerker preconditioner as scf_kerker, skipped when the step has stopped the SCF
+static int enqueue_kerker(sq_ctx *c, const double *u, double *y, double k0, bool eager)
+{
+    auto &F = c->scf;
+    const System &S = c->sys;
+    int G[3];
+    if (int rc = cell_dims(c, G)) return rc;
+    const StencilCoeffs sc = stencil_of(S);
+    const long n = F.ng;
+    const double shift = 0.5 * k0 * k0;
+    hipStream_t st = c->stream;
+    ScfCtl *ctl = F.ctl;
+    const int *cv = &ctl->conv;
+    HIP_TRY(launch_lap_periodic(u, F.r, G, S.order, sc, st, cv));
+    HIP_TRY(launch_zero(y, n, cv, st));
+    HIP_TRY(launch_copy(F.p, F.r, n, cv, st));
+    HIP_TRY(launch_dot(F.r, F.r, n, 1.0, F.part, kRedBlocks, F.s, st, cv));
+    HIP_TRY(launch_cg_begin(F.s, &ctl->kstop, &ctl->kdone, &ctl->kiters, 1e-20, 0, cv, st));
+    const int *sk = &ctl->kdone;
+    auto block = [&]() -> int {
+        for (int k = 0; k < 4; k++) {
+            HIP_TRY(launch_lap_periodic(F.p, F.Ap, G, S.order, sc, st, sk));
+            HIP_TRY(launch_axpbypgz(shift, F.p, 1.0, F.Ap, 0.0, nullptr, n, st, sk));
+            HIP_TRY(launch_dot(F.p, F.Ap, n, 1.0, F.part, kRedBlocks, F.s + 1, st, sk));
+            HIP_TRY(launch_cg_xr(y, F.r, F.p, F.Ap, F.s, n, st, sk));
+            HIP_TRY(launch_dot(F.r, F.r, n, 1.0, F.part, kRedBlocks, F.s + 2, st, sk));
+            HIP_TRY(launch_cg_p(F.p, F.r, F.s, n, st, sk));
+        }
+        HIP_TRY(launch_cg_check(F.s, &ctl->kstop, &ctl->kdone, &ctl->kiters, 2000, st));
+        return SQ_OK;
+    };
+    if (int rc = cg_blocks(c, eager, kKerkerBlocks, &ctl->kdone, block)) return rc;
+    HIP_TRY(launch_cg_end(F.s, &ctl->kstop, &ctl->kfail, cv, st));
+    return SQ_OK;
+}
+
+// The SQ pass of sq_run over every node, without its host checks (the first
+// bad node status lands in ctl->bad)
+static int enqueue_sq_pass(sq_ctx *c, int mode, int precision)
+{
+    const System &S = c->sys;
+    const long count = S.n_nodes();
+    hipStream_t st = c->stream;
+    c->last_exit = false;
+    if (c->zmarch && c->cfg_B[precision] > 1) {
+        const int B = c->cfg_B[precision];
+        KParams P = make_batch_params(c, mode, precision, B, c->batches, c->n_batches);
+        int grid = c->zb_grid[precision][B];
+        if (c->n_batches < grid) grid = std::max<long>(8, (c->n_batches + 7) & ~7L);
+        c->last_exit = P.exit_tol > 0;
+        HIP_TRY(launch_zero_int(c->queue, 8, st));
+        HIP_TRY(launch_zbatch(P, precision, B, c->cfg_ug[precision], grid, c->zb_lds[B], st));
+    } else if (c->zmarch) {
+        KParams P = make_params(c, mode, precision, 0, count, true);
+        int grid = c->zgrid[precision];
+        if (count < grid) grid = std::max<long>(8, (count + 7) & ~7L);
+        c->last_exit = P.exit_tol > 0;
+        HIP_TRY(launch_zero_int(c->queue, 8, st));
+        HIP_TRY(launch_zmarch(P, precision, grid, c->zlds, st));
+    } else {
+        KParams P = make_params(c, mode, precision, 0, count);
+        HIP_TRY(launch_lanczos(P, precision, (int)std::min<long>(c->grid, count), c->lds, st));
+    }
+    HIP_TRY(launch_status_any(c->status, count, c->scf.ctl, st));
+    HIP_TRY(launch_ql(c->a, c->b, S.nqp, S.nqp + 1, count, c->lam, c->w, c->lmin, c->lmax, c->status, c->qlws, st,
+                      c->last_exit ? c->nsteps : nullptr));
+    HIP_TRY(launch_status_any(c->status, count, c->scf.ctl, st));
+    return SQ_OK;
+}
+
+// Fermi level by bisection, then density, band energy and -TS (density_device's steps)
+static int enqueue_fermi_density(sq_ctx *c, double *rho_out)
+{
+    const System &S = c->sys;
+    const long nn = S.n_nodes(), nq = nn * S.nqp;
+    const double beta = S.beta, nelec = S.noetot;
+    hipStream_t st = c->stream;
+    ScfCtl *ctl = c->scf.ctl;
+    HIP_TRY(launch_lam_range(c->lmin, c->lmax, nn, c->out, st));
+    HIP_TRY(launch_bis_begin(c->out, ctl, st));
+    HIP_TRY(launch_occupation_partial_dev(c->lam, c->w, nq, beta, &ctl->lo, nullptr, c->partial, kRedBlocks, st));
+    HIP_TRY(launch_bis_store(c->partial, kRedBlocks, &ctl->glo, st));
+    HIP_TRY(launch_occupation_partial_dev(c->lam, c->w, nq, beta, &ctl->hi, nullptr, c->partial, kRedBlocks, st));
+    HIP_TRY(launch_bis_store(c->partial, kRedBlocks, &ctl->ghi, st));
+    HIP_TRY(launch_bis_first(ctl, nelec, st));
+    for (int k = 0; k < 200; k++) {
+        HIP_TRY(launch_occupation_partial_dev(c->lam, c->w, nq, beta, &ctl->mid, &ctl->fdone, c->partial, kRedBlocks, st));
+        HIP_TRY(launch_bis_step(c->partial, kRedBlocks, ctl, nelec, st));
+    }
+    HIP_TRY(launch_bis_end(ctl, st));
+    HIP_TRY(launch_density_energy_dev(c->lam, c->w, nn, S.nqp, beta, &ctl->lf, 2.0 / S.dV, rho_out, c->partial,
+                                      c->partial2, kRedBlocks, c->out, st));
+    return SQ_OK;
+}
+
+// One whole SCF step: V_eff from the input density, SQ pass, Fermi level and
+// density, energies of the input density, residual and stop, Pulay history,
+// mixing into F.rho.
+static int enqueue_scf_step(sq_ctx *c, const StepPlan &pl)
+{
+    auto &F = c->scf;
+    const System &S = c->sys;
+    int G[3];
+    if (int rc = cell_dims(c, G)) return rc;
+    const long n = F.ng;
+    hipStream_t st = c->stream;
+    ScfCtl *ctl = F.ctl;
+    const int ext0[3] = {S.ext0[0], S.ext0[1], S.ext0[2]}, extn[3] = {S.extn[0], S.extn[1], S.extn[2]};
+    int rc;
+    HIP_TRY(launch_step_begin(ctl, st));
+    HIP_TRY(launch_xc(F.rho, n, F.eps, F.vxc, st));
+    HIP_TRY(launch_axpbypgz(1.0, F.rho, 0.0, F.veff, 1.0, F.b, n, st));   // rho + b
+    HIP_TRY(hipEventRecord(F.sev[0], st));
+    if ((rc = enqueue_poisson(c, F.veff, pl.tolp, pl.eager))) return rc;
+    HIP_TRY(hipEventRecord(F.sev[1], st));
+    HIP_TRY(launch_axpbypgz(1.0, F.phi, 0.0, F.veff, 1.0, F.vxc, n, st));  // phi + V_xc
+    HIP_TRY(launch_veff_to_diag(F.veff, S.cdiag, ext0, extn, G, c->diag64, c->diag32, st));
+    HIP_TRY(hipEventRecord(F.sev[2], st));
+    if ((rc = enqueue_sq_pass(c, pl.mode, pl.prec)) || (rc = enqueue_fermi_density(c, F.rho_out))) return rc;
+    HIP_TRY(hipEventRecord(F.sev[3], st));
+    // energies of the input density; residual f = rho_out - rho_in; stop
+    HIP_TRY(launch_dot(F.rho, F.eps, n, 1.0, F.part, kRedBlocks, F.s + 6, st));
+    HIP_TRY(launch_dot(F.rho, F.vxc, n, 1.0, F.part, kRedBlocks, F.s + 7, st));
+    HIP_TRY(launch_dot(F.phi, F.b, n, 1.0, F.part, kRedBlocks, F.s + 8, st));
+    HIP_TRY(launch_dot(F.phi, F.rho, n, 1.0, F.part, kRedBlocks, F.s + 9, st));
+    HIP_TRY(launch_axpbypgz(1.0, F.rho_out, 0.0, F.f, -1.0, F.rho, n, st));
+    HIP_TRY(launch_dot(F.f, F.f, n, 1.0, F.part, kRedBlocks, F.s + 10, st));
+    HIP_TRY(launch_dot(F.rho_out, F.rho_out, n, 1.0, F.part, kRedBlocks, F.s + 11, st));
+    HIP_TRY(launch_conv(F.s, ctl, pl.tol, st));
+    // Periodic Pulay: history of iterate and residual differences, Gram system, solve
+    const int *cv = &ctl->conv;
+    HIP_TRY(launch_hist_push(F.rho, F.x_prev, F.f, F.f_prev, F.dX, F.dF, n, ctl, st));
+    HIP_TRY(launch_hist_rotate(ctl, pl.m, pl.pf, st));
+    HIP_TRY(launch_copy(F.x_prev, F.rho, n, cv, st));
+    HIP_TRY(launch_copy(F.f_prev, F.f, n, cv, st));
+    for (int a = 0; a < pl.m; a++) {
+        for (int b2 = 0; b2 <= a; b2++)
+            HIP_TRY(launch_dot(F.dF + (size_t)a * n, F.dF + (size_t)b2 * n, n, 1.0, F.part, kRedBlocks,
+                               &ctl->gram[a * pl.m + b2], st, &ctl->gskip[a]));
+        HIP_TRY(launch_dot(F.dF + (size_t)a * n, F.f, n, 1.0, F.part, kRedBlocks, &ctl->rhs[a], st, &ctl->gskip[a]));
+    }
+    HIP_TRY(launch_pulay_solve(ctl, pl.m, st));
+    if (pl.kerker > 0) {
+        // x_new = (x - dX g) + beta P (f - dF g)
+        HIP_TRY(launch_pulay_split(F.rho, F.f, F.dX, F.dF, ctl->g, 0, n, F.rho_out, F.ku, st, &ctl->mu, cv));
+        if ((rc = enqueue_kerker(c, F.ku, F.ky, pl.kerker, pl.eager))) return rc;
+        HIP_TRY(launch_axpbypgz(pl.beta, F.ky, 1.0, F.rho_out, 0.0, nullptr, n, st, cv));
+    } else {
+        HIP_TRY(launch_pulay_update(F.rho, F.f, F.dX, F.dF, ctl->g, 0, pl.beta, n, F.rho_out, st, &ctl->mu, cv));
+    }
+    HIP_TRY(launch_copy(F.rho, F.rho_out, n, cv, st));
+    HIP_TRY(launch_it_next(ctl, st));
+    return SQ_OK;
 }
 
 void sq_default_scf_options(sq_scf_options *o)
@@ -1460,22 +1676,36 @@ int sq_scf(sq_ctx *c, const sq_scf_options *opt_in, sq_scf_result *res)
     }
     HIP_TRY(hipMemset(F.phi, 0, n * sizeof(double)));
     memset(res, 0, sizeof *res);
+    if (m > kMaxHist) return set_err(SQ_ERR_ARG, "MIXING_HISTORY above the supported maximum (16)");
     double t_sq = 0.0, t_poisson = 0.0, rule_sum = 0.0;
-    int nhist = 0, head = 0;
-    const int ext0[3] = {S.ext0[0], S.ext0[1], S.ext0[2]}, extn[3] = {S.extn[0], S.extn[1], S.extn[2]};
+    // the control block: iteration 1, empty Pulay history
+    {
+        ScfCtl init;
+        memset(&init, 0, sizeof init);
+        init.it = 1;
+        init.bad = -1;
+        HIP_TRY(hipMemcpy(F.ctl, &init, sizeof init, hipMemcpyHostToDevice));
+    }
+    // The step is captured as one graph after the first (eager) step, unless
+    // SQ_NO_GRAPHS is set or the per-node early exit (which takes the previous
+    // Fermi level from the host) is on.
+    bool use_graph = getenv("SQ_NO_GRAPHS") == nullptr && !(opt.exit_tol > 0);
+    if (c->zmarch && c->cfg_B[opt.precision] > 1) {
+        if ((rc = ensure_batches(c, c->cfg_B[opt.precision], 0, n))) return rc;
+    }
+    auto drop_graph = [&]() {
+        if (F.step_exec) (void)hipGraphExecDestroy(F.step_exec);
+        if (F.step_graph) (void)hipGraphDestroy(F.step_graph);
+        F.step_exec = nullptr;
+        F.step_graph = nullptr;
+    };
+    drop_graph();
+    struct GraphGuard {
+        std::function<void()> f;
+        ~GraphGuard() { f(); }
+    } gguard{drop_graph};
+    int graph_steps = 0;
     for (int it = 1; it <= maxit; it++) {
-        // V_eff pipeline
-        HIP_TRY(launch_xc(F.rho, n, F.eps, F.vxc, c->stream));
-        HIP_TRY(launch_axpbypgz(1.0, F.rho, 0.0, F.veff, 1.0, F.b, n, c->stream));   // rho + b
-        const auto tp = std::chrono::steady_clock::now();
-        int pits = 0;
-        if ((rc = scf_poisson(c, F.veff, tolp, 20000, &pits))) return rc;
-        t_poisson += std::chrono::duration<double>(std::chrono::steady_clock::now() - tp).count();
-        HIP_TRY(launch_axpbypgz(1.0, F.phi, 0.0, F.veff, 1.0, F.vxc, n, c->stream));  // phi + V_xc
-        HIP_TRY(launch_veff_to_diag(F.veff, S.cdiag, ext0, extn, G, c->diag64, c->diag32, c->stream));
-        // SQ pass (from the second iteration with the per-node early exit at the
-        // previous Fermi level, when asked for) and density
-        const auto ts = std::chrono::steady_clock::now();
         if (opt.exit_tol > 0 && it > 1)
             sq_set_early_exit(c, opt.exit_tol, res->lambda_f, 0.0, opt.exit_min > 0 ? opt.exit_min : 30,
                               opt.exit_every > 0 ? opt.exit_every : 5);
@@ -1485,31 +1715,70 @@ int sq_scf(sq_ctx *c, const sq_scf_options *opt_in, sq_scf_result *res)
         F.last_exit_beta = c->exit_beta;
         F.last_exit_min = c->exit_min;
         F.last_exit_every = c->exit_every;
-        rc = sq_run(c, opt.nonlocal_mode, opt.precision, 0, n);
+        StepPlan pl{opt.nonlocal_mode, opt.precision, beta, tol, tolp, opt.kerker, m, pf, !(use_graph && it > 1)};
+        if (!pl.eager && !F.step_exec) {
+            // capture once; a capture the runtime refuses leaves the loop eager
+            HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+            rc = enqueue_scf_step(c, pl);
+            hipGraph_t g = nullptr;
+            const hipError_t e = hipStreamEndCapture(c->stream, &g);
+            if (!rc && e == hipSuccess && g && hipGraphInstantiate(&F.step_exec, g, nullptr, nullptr, 0) == hipSuccess) {
+                F.step_graph = g;
+            } else {
+                if (g) (void)hipGraphDestroy(g);
+                F.step_exec = nullptr;
+                (void)hipGetLastError();
+                use_graph = false;
+                pl.eager = true;
+                rc = SQ_OK;
+            }
+        }
+        if (pl.eager) {
+            rc = enqueue_scf_step(c, pl);
+        } else {
+            HIP_TRY(hipGraphLaunch(F.step_exec, c->stream));
+            graph_steps++;
+        }
         c->exit_tol = 0.0;
         if (rc) return rc;
-        {
+        // the step's scalars: control block, dot slots, band energy and -TS
+        ScfCtl h;
+        double sl[16], eo[2];
+        HIP_TRY(hipMemcpyAsync(&h, F.ctl, sizeof h, hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(hipMemcpyAsync(sl, F.s, sizeof sl, hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(hipMemcpyAsync(eo, c->out, sizeof eo, hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        float ms = 0.f;
+        HIP_TRY(hipEventElapsedTime(&ms, F.sev[0], F.sev[1]));
+        t_poisson += ms * 1e-3;
+        HIP_TRY(hipEventElapsedTime(&ms, F.sev[2], F.sev[3]));
+        t_sq += ms * 1e-3;
+        if (h.bad >= 0) {
+            int stv = 0;
+            HIP_TRY(hipMemcpy(&stv, c->status + h.bad, sizeof(int), hipMemcpyDeviceToHost));
+            char msg[160];
+            snprintf(msg, sizeof msg, "SCF pass: node %d (rank-box id) status %d", h.bad, stv);
+            return set_err(stv == 7 ? SQ_ERR_BREAKDOWN : stv == 5 ? SQ_ERR_QL : SQ_ERR_ARG, msg);
+        }
+        if (h.pfail) return set_err(SQ_ERR_RANGE, "Poisson: conjugate gradients did not reach TOL_POISSON");
+        if (h.fbad) return set_err(SQ_ERR_RANGE, "Fermi level not bracketed by the spectrum");
+        if (h.kfail) return set_err(SQ_ERR_RANGE, "Kerker: conjugate gradients did not converge");
+        c->have_run = true;
+        c->last_begin = 0;
+        c->last_count = n;
+        if (c->last_exit) {
             std::vector<int> ns(n);
             if ((rc = sq_get_rule_sizes(c, ns.data()))) return rc;
             long tot = 0;
             for (int v : ns) tot += v;
             rule_sum += (double)tot / n;
-        }
-        double lf, eb, mts;
-        if ((rc = density_device(c, 0.0, 0.0, &lf, F.rho_out, &eb, &mts))) return rc;
-        t_sq += std::chrono::duration<double>(std::chrono::steady_clock::now() - ts).count();
+        } else rule_sum += S.nqp;
+        const double lf = h.lf, eb = eo[0], mts = eo[1];
         F.lambda_f = lf;
         F.last_mode = opt.nonlocal_mode;
         F.last_prec = opt.precision;
         F.force_ready = true;
-        // energies from the input density
-        double exc, evxc, pb, prho, rr, oo;
-        if ((rc = scf_dot(c, F.rho, F.eps, 6, &exc)) || (rc = scf_dot(c, F.rho, F.vxc, 7, &evxc)) ||
-            (rc = scf_dot(c, F.phi, F.b, 8, &pb)) || (rc = scf_dot(c, F.phi, F.rho, 9, &prho)))
-            return rc;
-        // residual f = rho_out - rho_in
-        HIP_TRY(launch_axpbypgz(1.0, F.rho_out, 0.0, F.f, -1.0, F.rho, n, c->stream));
-        if ((rc = scf_dot(c, F.f, F.f, 10, &rr)) || (rc = scf_dot(c, F.rho_out, F.rho_out, 11, &oo))) return rc;
+        const double exc = sl[6], evxc = sl[7], pb = sl[8], prho = sl[9], rr = sl[10], oo = sl[11];
         res->iterations = it;
         res->residual = std::sqrt(rr / oo);
         res->lambda_f = lf;
@@ -1520,45 +1789,17 @@ int sq_scf(sq_ctx *c, const sq_scf_options *opt_in, sq_scf_result *res)
         res->e_el = 0.5 * (pb - prho) * dV;
         res->e_self = F.eself;
         res->e_total = eb + res->e_xc - res->e_vxc + res->e_el - F.eself + mts;
-        res->poisson_iterations = pits;
+        res->poisson_iterations = h.piters;
+        res->graph_steps = graph_steps;
+        if (opt.tol_report > 0 && res->residual < opt.tol_report && res->report_iterations == 0) {
+            res->report_iterations = it;
+            res->report_wall_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+            res->report_e_total = res->e_total;
+        }
         if (opt.verbose)
-            fprintf(stderr, "scf %3d  residual %.3e  E %.12f  lambda_f %.10f  poisson %d\n", it, res->residual, res->e_total, lf, pits);
-        if (res->residual < tol) { res->converged = 1; break; }
-        // Periodic Pulay: history of iterate and residual differences
-        if (it > 1) {
-            double *dx = F.dX + (size_t)head * n, *df = F.dF + (size_t)head * n;
-            HIP_TRY(launch_axpbypgz(1.0, F.rho, 0.0, dx, -1.0, F.x_prev, n, c->stream));
-            HIP_TRY(launch_axpbypgz(1.0, F.f, 0.0, df, -1.0, F.f_prev, n, c->stream));
-            head = (head + 1) % m;
-            nhist = std::min(nhist + 1, m);
-        }
-        HIP_TRY(hipMemcpyAsync(F.x_prev, F.rho, n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
-        HIP_TRY(hipMemcpyAsync(F.f_prev, F.f, n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
-        int mu = 0;
-        if (it % pf == 0 && nhist > 0) {
-            std::vector<double> A((size_t)nhist * nhist), rhs(nhist);
-            for (int a = 0; a < nhist; a++) {
-                for (int b2 = 0; b2 <= a; b2++) {
-                    double v;
-                    if ((rc = scf_dot(c, F.dF + (size_t)a * n, F.dF + (size_t)b2 * n, 12, &v))) return rc;
-                    A[a * nhist + b2] = A[b2 * nhist + a] = v;
-                }
-                if ((rc = scf_dot(c, F.dF + (size_t)a * n, F.f, 12, &rhs[a]))) return rc;
-            }
-            if (small_solve(A, rhs, nhist)) {
-                HIP_TRY(hipMemcpy(F.g, rhs.data(), nhist * sizeof(double), hipMemcpyHostToDevice));
-                mu = nhist;
-            }
-        }
-        if (opt.kerker > 0) {
-            // x_new = (x - dX g) + beta P (f - dF g)
-            HIP_TRY(launch_pulay_split(F.rho, F.f, F.dX, F.dF, F.g, mu, n, F.rho_out, F.ku, c->stream));
-            if ((rc = scf_kerker(c, F.ku, F.ky, opt.kerker, nullptr))) return rc;
-            HIP_TRY(launch_axpbypgz(beta, F.ky, 1.0, F.rho_out, 0.0, nullptr, n, c->stream));
-        } else {
-            HIP_TRY(launch_pulay_update(F.rho, F.f, F.dX, F.dF, F.g, mu, beta, n, F.rho_out, c->stream));
-        }
-        std::swap(F.rho, F.rho_out);   // rho_out now holds the mixed density; keep rho as the input
+            fprintf(stderr, "scf %3d  residual %.3e  E %.12f  lambda_f %.10f  poisson %d%s\n", it, res->residual,
+                    res->e_total, lf, h.piters, pl.eager ? "" : "  (graph)");
+        if (h.conv) { res->converged = 1; break; }
     }
     HIP_TRY(hipStreamSynchronize(c->stream));
     res->wall_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();

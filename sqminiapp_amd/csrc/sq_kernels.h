@@ -90,6 +90,12 @@ hipError_t launch_density_energy(const double *lam, const double *w, long n_node
                                  double two_over_dv, double *rho, double *part_e, double *part_s, int nblocks,
                                  double *out2, hipStream_t st);
 hipError_t launch_lam_range(const double *lmin, const double *lmax, long n, double *out, hipStream_t st);
+// the same with the Fermi level read from device memory (and a skip flag)
+hipError_t launch_occupation_partial_dev(const double *lam, const double *w, long nq, double beta, const double *lf_dev,
+                                         const int *skip, double *partial, int nblocks, hipStream_t st);
+hipError_t launch_density_energy_dev(const double *lam, const double *w, long n_nodes, int nqp, double beta,
+                                     const double *lf_dev, double two_over_dv, double *rho, double *part_e, double *part_s,
+                                     int nblocks, double *out2, hipStream_t st);
 
 // ---- forces (sq_forces.hip)
 // coef[r][k] = (f(T) e_1)_k for node ids[r]'s tridiagonal (a, b at id * stride,
@@ -140,17 +146,60 @@ struct StencilCoeffs {
     double w[3][11];   // -(1/2) Lap_h weights per axis, w[d][0] the centre
 };
 hipError_t launch_xc(const double *rho, long n, double *eps, double *vxc, hipStream_t st);
-hipError_t launch_lap_periodic(const double *x, double *y, const int G[3], int order, const StencilCoeffs &c, hipStream_t st);
+// skip (device int, may be null): a kernel whose *skip is nonzero returns at once, so a
+// captured sequence can hold steps that a device-side test has made unnecessary
+hipError_t launch_lap_periodic(const double *x, double *y, const int G[3], int order, const StencilCoeffs &c, hipStream_t st,
+                               const int *skip = nullptr);
 hipError_t launch_dot(const double *a, const double *b, long n, double scale, double *partial, int nblocks, double *out,
-                      hipStream_t st);
+                      hipStream_t st, const int *skip = nullptr);
 hipError_t launch_axpbypgz(double alpha, const double *x, double beta, double *y, double gamma, const double *z, long n,
-                           hipStream_t st);
-hipError_t launch_cg_xr(double *x, double *r, const double *p, const double *Ap, const double *s, long n, hipStream_t st);
-hipError_t launch_cg_p(double *p, const double *r, double *s, long n, hipStream_t st);
+                           hipStream_t st, const int *skip = nullptr);
+hipError_t launch_cg_xr(double *x, double *r, const double *p, const double *Ap, const double *s, long n, hipStream_t st,
+                        const int *skip = nullptr);
+hipError_t launch_cg_p(double *p, const double *r, double *s, long n, hipStream_t st, const int *skip = nullptr);
+// m_dev (device int, may be null) replaces m
 hipError_t launch_pulay_update(const double *x, const double *f, const double *dX, const double *dF, const double *g, int m,
-                               double beta, long n, double *xn, hipStream_t st);
+                               double beta, long n, double *xn, hipStream_t st, const int *m_dev = nullptr,
+                               const int *skip = nullptr);
 hipError_t launch_pulay_split(const double *x, const double *f, const double *dX, const double *dF, const double *g, int m,
-                              long n, double *xa, double *u, hipStream_t st);
+                              long n, double *xa, double *u, hipStream_t st, const int *m_dev = nullptr,
+                              const int *skip = nullptr);
+// ---- device-side decisions of one SCF step (sq_scfstep.hip)
+constexpr int kMaxHist = 16;   // Pulay history limit of the device solve
+struct ScfCtl {
+    double pstop, kstop;                 // CG thresholds: Poisson, Kerker
+    double lo, hi, mid, lf, glo, ghi;    // Fermi bisection
+    double gram[kMaxHist * kMaxHist], rhs[kMaxHist], g[kMaxHist];   // Pulay Gram system, coefficients
+    int pdone, piters, pfail;            // Poisson CG: stopped, iterations, not converged
+    int kdone, kiters, kfail;            // Kerker CG
+    int fdone, fbad, fcount;             // bisection: stopped, level not bracketed, counts
+    int it, head, nhist, mu;             // SCF iteration; Pulay history slot, length, coefficients used
+    int conv, nopush, pulay, bad;        // SCF stopped; no history push; Pulay step; first bad node (-1)
+    int gskip[kMaxHist];                 // Gram row a not needed this step
+};
+hipError_t launch_cg_begin(const double *s, double *stop, int *done, int *iters, double scale, int sidx, const int *skip,
+                           hipStream_t st);
+hipError_t launch_cg_check(const double *s, const double *stop, int *done, int *iters, int maxit, hipStream_t st);
+hipError_t launch_cg_end(const double *s, const double *stop, int *fail, const int *skip, hipStream_t st);
+hipError_t launch_axpb_mean(double alpha, const double *x, double *y, double mscale, const double *sum, long n,
+                            const int *skip, hipStream_t st);
+hipError_t launch_copy(double *dst, const double *src, long n, const int *skip, hipStream_t st);
+hipError_t launch_zero(double *y, long n, const int *skip, hipStream_t st);
+hipError_t launch_zero_int(int *y, int n, hipStream_t st);
+hipError_t launch_bis_begin(const double *range, ScfCtl *c, hipStream_t st);
+hipError_t launch_bis_store(const double *partial, int nblocks, double *out, hipStream_t st);
+hipError_t launch_bis_first(ScfCtl *c, double nelec, hipStream_t st);
+hipError_t launch_bis_step(const double *partial, int nblocks, ScfCtl *c, double nelec, hipStream_t st);
+hipError_t launch_bis_end(ScfCtl *c, hipStream_t st);
+hipError_t launch_status_any(const int *status, long n, ScfCtl *c, hipStream_t st);
+hipError_t launch_conv(const double *s, ScfCtl *c, double tol, hipStream_t st);
+hipError_t launch_hist_push(const double *x, const double *xp, const double *f, const double *fp, double *dX, double *dF,
+                            long n, const ScfCtl *c, hipStream_t st);
+hipError_t launch_hist_rotate(ScfCtl *c, int m, int pf, hipStream_t st);
+hipError_t launch_pulay_solve(ScfCtl *c, int m, hipStream_t st);
+hipError_t launch_it_next(ScfCtl *c, hipStream_t st);
+hipError_t launch_step_begin(ScfCtl *c, hipStream_t st);
+
 hipError_t launch_veff_to_diag(const double *veff, double cdiag, const int ext0[3], const int extn[3], const int G[3],
                                double *d64, float *d32, hipStream_t st);
 

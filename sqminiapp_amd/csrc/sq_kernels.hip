@@ -315,8 +315,11 @@ constexpr int kRedThreads = 256;
 
 // partial[block] = sum over the block's chunk of w * f(beta (lam - lf))
 __global__ void __launch_bounds__(kRedThreads) occupation_partial(const double *lam, const double *w, long nq, double beta,
-                                                                  double lf, double *partial)
+                                                                  double lf, double *partial, const double *lf_dev,
+                                                                  const int *skip)
 {
+    if (skip && *skip) return;
+    if (lf_dev) lf = *lf_dev;
     __shared__ double red[kRedThreads / 64];
     const long chunk = (nq + gridDim.x - 1) / gridDim.x;
     const long q0 = blockIdx.x * chunk, q1 = min(nq, q0 + chunk);
@@ -345,8 +348,9 @@ __global__ void ordered_sum(const double *partial, int n, double scale, double *
 // per node: rho = (2/dV) sum_i w f; partials of sum w lam f and sum w S(f)
 __global__ void __launch_bounds__(kRedThreads) density_energy(const double *lam, const double *w, long n_nodes, int nqp,
                                                               double beta, double lf, double two_over_dv, double *rho,
-                                                              double *part_e, double *part_s)
+                                                              double *part_e, double *part_s, const double *lf_dev)
 {
+    if (lf_dev) lf = *lf_dev;
     __shared__ double re[kRedThreads / 64], rs[kRedThreads / 64];
     const long chunk = (n_nodes + gridDim.x - 1) / gridDim.x;
     const long p0 = blockIdx.x * chunk, p1 = min(n_nodes, p0 + chunk);
@@ -438,7 +442,8 @@ hipError_t launch_ql(const double *a, const double *b, int n, int stride, long c
 hipError_t launch_occupation(const double *lam, const double *w, long nq, double beta, double lf, double *partial,
                              int nblocks, double *out, hipStream_t st)
 {
-    hipLaunchKernelGGL(occupation_partial, dim3(nblocks), dim3(kRedThreads), 0, st, lam, w, nq, beta, lf, partial);
+    hipLaunchKernelGGL(occupation_partial, dim3(nblocks), dim3(kRedThreads), 0, st, lam, w, nq, beta, lf, partial,
+                       (const double *)nullptr, (const int *)nullptr);
     hipLaunchKernelGGL(ordered_sum, dim3(1), dim3(64), 0, st, partial, nblocks, 2.0, out);
     return hipGetLastError();
 }
@@ -448,7 +453,26 @@ hipError_t launch_density_energy(const double *lam, const double *w, long n_node
                                  double *out2, hipStream_t st)
 {
     hipLaunchKernelGGL(density_energy, dim3(nblocks), dim3(kRedThreads), 0, st, lam, w, n_nodes, nqp, beta, lf,
-                       two_over_dv, rho, part_e, part_s);
+                       two_over_dv, rho, part_e, part_s, (const double *)nullptr);
+    hipLaunchKernelGGL(ordered_sum, dim3(1), dim3(64), 0, st, part_e, nblocks, 2.0, out2);
+    hipLaunchKernelGGL(ordered_sum, dim3(1), dim3(64), 0, st, part_s, nblocks, 2.0 / beta, out2 + 1);
+    return hipGetLastError();
+}
+
+hipError_t launch_occupation_partial_dev(const double *lam, const double *w, long nq, double beta, const double *lf_dev,
+                                         const int *skip, double *partial, int nblocks, hipStream_t st)
+{
+    hipLaunchKernelGGL(occupation_partial, dim3(nblocks), dim3(kRedThreads), 0, st, lam, w, nq, beta, 0.0, partial, lf_dev,
+                       skip);
+    return hipGetLastError();
+}
+
+hipError_t launch_density_energy_dev(const double *lam, const double *w, long n_nodes, int nqp, double beta,
+                                     const double *lf_dev, double two_over_dv, double *rho, double *part_e, double *part_s,
+                                     int nblocks, double *out2, hipStream_t st)
+{
+    hipLaunchKernelGGL(density_energy, dim3(nblocks), dim3(kRedThreads), 0, st, lam, w, n_nodes, nqp, beta, 0.0,
+                       two_over_dv, rho, part_e, part_s, lf_dev);
     hipLaunchKernelGGL(ordered_sum, dim3(1), dim3(64), 0, st, part_e, nblocks, 2.0, out2);
     hipLaunchKernelGGL(ordered_sum, dim3(1), dim3(64), 0, st, part_s, nblocks, 2.0 / beta, out2 + 1);
     return hipGetLastError();

@@ -59,8 +59,9 @@ __global__ void __launch_bounds__(kT) xc_lda_pw(const double *__restrict__ rho, 
 // Poisson operator, Create_Laplacian, lsdft_initObjs.cc:309-388, on a periodic
 // DMDA): y = (c0x+c0y+c0z) x + sum_a c_a (x(-a) + x(+a)) per axis.
 __global__ void __launch_bounds__(kT) lap_periodic(const double *__restrict__ x, double *__restrict__ y, int G0, int G1,
-                                                    int G2, int order, StencilCoeffs c)
+                                                    int G2, int order, StencilCoeffs c, const int *skip)
 {
+    if (skip && *skip) return;
     const long n = (long)G0 * G1 * G2;
     for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) {
         const int i = (int)(q % G0);
@@ -81,8 +82,9 @@ __global__ void __launch_bounds__(kT) lap_periodic(const double *__restrict__ x,
 
 // Block partials of sum a*b over fixed contiguous chunks (b == nullptr: sum a).
 __global__ void __launch_bounds__(kT) dot_partial(const double *__restrict__ a, const double *__restrict__ b, long n,
-                                                  double *__restrict__ partial)
+                                                  double *__restrict__ partial, const int *skip)
 {
+    if (skip && *skip) return;
     __shared__ double sh[kT / 64];
     const long chunk = (n + gridDim.x - 1) / gridDim.x;
     const long p0 = blockIdx.x * chunk, p1 = min(n, p0 + chunk);
@@ -93,8 +95,10 @@ __global__ void __launch_bounds__(kT) dot_partial(const double *__restrict__ a, 
 }
 
 // out[slot] = scale * sum of the partials, in block order.
-__global__ void sum_partials(const double *__restrict__ partial, int n, double scale, double *__restrict__ out)
+__global__ void sum_partials(const double *__restrict__ partial, int n, double scale, double *__restrict__ out,
+                             const int *skip)
 {
+    if (skip && *skip) return;
     if (threadIdx.x == 0 && blockIdx.x == 0) {
         double s = 0.0;
         for (int k = 0; k < n; k++) s += partial[k];
@@ -105,8 +109,9 @@ __global__ void sum_partials(const double *__restrict__ partial, int n, double s
 // y = alpha x + gamma z + beta y; with z null the constant gamma is added;
 // y is not read when beta is 0 (x may alias y)
 __global__ void __launch_bounds__(kT) axpbypgz(double alpha, const double *x, double beta, double *y, double gamma,
-                                               const double *__restrict__ z, long n)
+                                               const double *__restrict__ z, long n, const int *skip)
 {
+    if (skip && *skip) return;
     for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) {
         double v = alpha * x[q] + (z ? gamma * z[q] : gamma);
         if (beta != 0.0) v = v + beta * y[q];
@@ -117,8 +122,10 @@ __global__ void __launch_bounds__(kT) axpbypgz(double alpha, const double *x, do
 // Conjugate-gradient step with the scalars on the device: s = {rr, pAp, rr_new, ...}.
 // x += (rr/pAp) p ; r -= (rr/pAp) Ap
 __global__ void __launch_bounds__(kT) cg_xr(double *__restrict__ x, double *__restrict__ r, const double *__restrict__ p,
-                                            const double *__restrict__ Ap, const double *__restrict__ s, long n)
+                                            const double *__restrict__ Ap, const double *__restrict__ s, long n,
+                                            const int *skip)
 {
+    if (skip && *skip) return;
     const double alpha = s[0] / s[1];
     for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) {
         x[q] = fma(alpha, p[q], x[q]);
@@ -127,14 +134,17 @@ __global__ void __launch_bounds__(kT) cg_xr(double *__restrict__ x, double *__re
 }
 
 // p = r + (rr_new/rr) p, then rr := rr_new
-__global__ void __launch_bounds__(kT) cg_p(double *__restrict__ p, const double *__restrict__ r, double *__restrict__ s, long n)
+__global__ void __launch_bounds__(kT) cg_p(double *__restrict__ p, const double *__restrict__ r, double *__restrict__ s, long n,
+                                           const int *skip)
 {
+    if (skip && *skip) return;
     const double beta = s[2] / s[0];
     for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x)
         p[q] = fma(beta, p[q], r[q]);
 }
-__global__ void cg_roll(double *s)
+__global__ void cg_roll(double *s, const int *skip)
 {
+    if (skip && *skip) return;
     if (threadIdx.x == 0 && blockIdx.x == 0) s[0] = s[2];
 }
 
@@ -143,8 +153,10 @@ __global__ void cg_roll(double *s)
 __global__ void __launch_bounds__(kT) pulay_update(const double *__restrict__ x, const double *__restrict__ f,
                                                    const double *__restrict__ dX, const double *__restrict__ dF,
                                                    const double *__restrict__ g, int m, double beta, long n,
-                                                   double *__restrict__ xn)
+                                                   double *__restrict__ xn, const int *m_dev, const int *skip)
 {
+    if (skip && *skip) return;
+    if (m_dev) m = *m_dev;
     for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) {
         double v = x[q] + beta * f[q];
         for (int j = 0; j < m; j++) v -= g[j] * (dX[(long)j * n + q] + beta * dF[(long)j * n + q]);
@@ -158,8 +170,10 @@ __global__ void __launch_bounds__(kT) pulay_update(const double *__restrict__ x,
 __global__ void __launch_bounds__(kT) pulay_split(const double *__restrict__ x, const double *__restrict__ f,
                                                   const double *__restrict__ dX, const double *__restrict__ dF,
                                                   const double *__restrict__ g, int m, long n, double *__restrict__ xa,
-                                                  double *__restrict__ u)
+                                                  double *__restrict__ u, const int *m_dev, const int *skip)
 {
+    if (skip && *skip) return;
+    if (m_dev) m = *m_dev;
     for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) {
         double a = x[q], b = f[q];
         for (int j = 0; j < m; j++) {
@@ -196,52 +210,54 @@ hipError_t launch_xc(const double *rho, long n, double *eps, double *vxc, hipStr
     return hipGetLastError();
 }
 
-hipError_t launch_lap_periodic(const double *x, double *y, const int G[3], int order, const StencilCoeffs &c, hipStream_t st)
+hipError_t launch_lap_periodic(const double *x, double *y, const int G[3], int order, const StencilCoeffs &c, hipStream_t st,
+                               const int *skip)
 {
     const long n = (long)G[0] * G[1] * G[2];
-    hipLaunchKernelGGL(lap_periodic, grid_for(n), dim3(kT), 0, st, x, y, G[0], G[1], G[2], order, c);
+    hipLaunchKernelGGL(lap_periodic, grid_for(n), dim3(kT), 0, st, x, y, G[0], G[1], G[2], order, c, skip);
     return hipGetLastError();
 }
 
 hipError_t launch_dot(const double *a, const double *b, long n, double scale, double *partial, int nblocks, double *out,
-                      hipStream_t st)
+                      hipStream_t st, const int *skip)
 {
-    hipLaunchKernelGGL(dot_partial, dim3(nblocks), dim3(kT), 0, st, a, b, n, partial);
-    hipLaunchKernelGGL(sum_partials, dim3(1), dim3(64), 0, st, partial, nblocks, scale, out);
+    hipLaunchKernelGGL(dot_partial, dim3(nblocks), dim3(kT), 0, st, a, b, n, partial, skip);
+    hipLaunchKernelGGL(sum_partials, dim3(1), dim3(64), 0, st, partial, nblocks, scale, out, skip);
     return hipGetLastError();
 }
 
 hipError_t launch_axpbypgz(double alpha, const double *x, double beta, double *y, double gamma, const double *z, long n,
-                           hipStream_t st)
+                           hipStream_t st, const int *skip)
 {
-    hipLaunchKernelGGL(axpbypgz, grid_for(n), dim3(kT), 0, st, alpha, x, beta, y, gamma, z, n);
+    hipLaunchKernelGGL(axpbypgz, grid_for(n), dim3(kT), 0, st, alpha, x, beta, y, gamma, z, n, skip);
     return hipGetLastError();
 }
 
-hipError_t launch_cg_xr(double *x, double *r, const double *p, const double *Ap, const double *s, long n, hipStream_t st)
+hipError_t launch_cg_xr(double *x, double *r, const double *p, const double *Ap, const double *s, long n, hipStream_t st,
+                        const int *skip)
 {
-    hipLaunchKernelGGL(cg_xr, grid_for(n), dim3(kT), 0, st, x, r, p, Ap, s, n);
+    hipLaunchKernelGGL(cg_xr, grid_for(n), dim3(kT), 0, st, x, r, p, Ap, s, n, skip);
     return hipGetLastError();
 }
 
-hipError_t launch_cg_p(double *p, const double *r, double *s, long n, hipStream_t st)
+hipError_t launch_cg_p(double *p, const double *r, double *s, long n, hipStream_t st, const int *skip)
 {
-    hipLaunchKernelGGL(cg_p, grid_for(n), dim3(kT), 0, st, p, r, s, n);
-    hipLaunchKernelGGL(cg_roll, dim3(1), dim3(64), 0, st, s);
+    hipLaunchKernelGGL(cg_p, grid_for(n), dim3(kT), 0, st, p, r, s, n, skip);
+    hipLaunchKernelGGL(cg_roll, dim3(1), dim3(64), 0, st, s, skip);
     return hipGetLastError();
 }
 
 hipError_t launch_pulay_update(const double *x, const double *f, const double *dX, const double *dF, const double *g, int m,
-                               double beta, long n, double *xn, hipStream_t st)
+                               double beta, long n, double *xn, hipStream_t st, const int *m_dev, const int *skip)
 {
-    hipLaunchKernelGGL(pulay_update, grid_for(n), dim3(kT), 0, st, x, f, dX, dF, g, m, beta, n, xn);
+    hipLaunchKernelGGL(pulay_update, grid_for(n), dim3(kT), 0, st, x, f, dX, dF, g, m, beta, n, xn, m_dev, skip);
     return hipGetLastError();
 }
 
 hipError_t launch_pulay_split(const double *x, const double *f, const double *dX, const double *dF, const double *g, int m,
-                              long n, double *xa, double *u, hipStream_t st)
+                              long n, double *xa, double *u, hipStream_t st, const int *m_dev, const int *skip)
 {
-    hipLaunchKernelGGL(pulay_split, grid_for(n), dim3(kT), 0, st, x, f, dX, dF, g, m, n, xa, u);
+    hipLaunchKernelGGL(pulay_split, grid_for(n), dim3(kT), 0, st, x, f, dX, dF, g, m, n, xa, u, m_dev, skip);
     return hipGetLastError();
 }
 

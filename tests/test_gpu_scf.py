@@ -159,3 +159,36 @@ def test_chosen_form_meets_the_accuracy_targets():
     assert abs(chosen["e_total"] - ref["e_total"]) / n < 1e-5
     assert np.max(np.abs(F - Fref)) < 1e-4
     assert chosen["iterations"] < plain["iterations"]
+
+
+@pytest.mark.parametrize("kerker,prec", [(0.0, sq.SQ_FP64), (1.0, sq.SQ_FP32)])
+def test_scf_step_graph_replays_match_launch_by_launch(monkeypatch, kerker, prec):
+    """The whole SCF step -- V_eff, Poisson CG, SQ pass, device Fermi bisection,
+    energies, Pulay history and solve, (Kerker) mixing -- captured once as a
+    graph and replayed from the second iteration on, against the same steps
+    launched one by one (SQ_NO_GRAPHS): the same iterations, energies and
+    densities, bit for bit; and the graph really replayed."""
+    kw = dict(tol=1e-7 if prec == sq.SQ_FP64 else 1e-5, max_iter=200, mixing=0.3 if kerker == 0 else 0.7, kerker=kerker)
+    with sq.Context(MG1, "Mg1", cell=True) as c:
+        monkeypatch.setenv("SQ_NO_GRAPHS", "1")
+        r_eager = c.scf(sq.SQ_NL_KB, prec, **kw)
+        rho_e = c.cell_field("rho")
+        monkeypatch.delenv("SQ_NO_GRAPHS")
+        r_graph = c.scf(sq.SQ_NL_KB, prec, **kw)
+        rho_g = c.cell_field("rho")
+    print("SCF eager / graph iterations", r_eager["iterations"], r_graph["iterations"], "graph steps",
+          r_graph["graph_steps"], flush=True)
+    assert r_eager["converged"] and r_graph["converged"]
+    assert r_eager["graph_steps"] == 0 and r_graph["graph_steps"] == r_graph["iterations"] - 1 > 0
+    assert r_graph["iterations"] == r_eager["iterations"] and r_graph["e_total"] == r_eager["e_total"]
+    assert r_graph["lambda_f"] == r_eager["lambda_f"] and np.array_equal(rho_g, rho_e)
+
+
+def test_scf_reports_the_looser_stop_of_the_same_run():
+    """tol_report records the first iteration below a looser residual: the
+    same as a run stopped there."""
+    with sq.Context(MG1, "Mg1", cell=True) as c:
+        r6 = c.scf(sq.SQ_NL_KB, sq.SQ_FP64, tol=1e-7, max_iter=200, mixing=0.3, tol_report=1e-5)
+        r5 = c.scf(sq.SQ_NL_KB, sq.SQ_FP64, tol=1e-5, max_iter=200, mixing=0.3)
+    assert r6["report_iterations"] == r5["iterations"] and r6["report_e_total"] == r5["e_total"]
+    assert 0 < r6["report_wall_s"] < r6["wall_s"]
