@@ -288,29 +288,35 @@ int sq_forces(sq_ctx *ctx, double *forces, double *f_local, double *f_nonlocal, 
  * No counterpart in the reference, whose SCF (declared only) would start from
  * the superposed atomic densities (lsdft_readfiles.cc:1026-1040).  A small
  * network predicts the SCF's correction to that superposition at every cell
- * point from 16 radial descriptors of the atoms within 6 bohr
- * (f_k = sum exp(-((r - mu_k)/s)^2) (cos(pi r/6) + 1)/2, mu_k = 0.4 k, s = 0.4);
- * 16 -> 64 -> 64 -> 1, tanh.  It is trained on densities the SQ SCF itself
- * converged (sq_mlp_add_sample after sq_scf) by the library's own forward and
- * backward kernels (Adam, full batch, FP32), and its guess goes to
- * sq_scf_options.rho0.  The superposition stays the fallback (rho0 = NULL). */
+ * point from 32 Behler-type symmetry functions of the Mg atoms within 8 bohr
+ * of the point: 16 radial Gaussians (centres 8k/15, width 8/15) and 16
+ * angular functions 2^(1-zeta) sum_{J<L} (1 + lam cos t_JL)^zeta
+ * exp(-eta (r_J^2 + r_L^2)) over eta in {0.005, 0.02}, zeta in {1, 2, 4, 16},
+ * lam = +-1, all with the cutoff (cos(pi r/8) + 1)/2; 32 -> 64 -> 64 -> 1,
+ * SiLU.  It is trained on densities the SQ SCF itself converged
+ * (sq_mlp_add_sample after sq_scf) by the library's own forward and backward
+ * kernels (Adam, full batch, FP32), and its guess (BF16 weights and
+ * activations, FP32 accumulation, or FP32) goes to sq_scf_options.rho0.  The
+ * superposition stays the fallback (rho0 = NULL). */
 typedef struct sq_mlp sq_mlp;
 
-/* parameter count (5313) and the normalization block: mean[16], 1/std[16], target scale */
-#define SQ_MLP_PARAMS 5313
-#define SQ_MLP_NORM 33
+/* parameter count (6337) and the normalization block: mean[32], 1/std[32], target scale */
+#define SQ_MLP_PARAMS 6337
+#define SQ_MLP_NORM 65
 
 int sq_mlp_create(int device, unsigned seed, sq_mlp **model);
 int sq_mlp_destroy(sq_mlp *model);
-/* params: SQ_MLP_PARAMS floats (W1[64][16], b1, W2[64][64], b2, w3, b3); norm:
+/* params: SQ_MLP_PARAMS floats (W1[64][32], b1, W2[64][64], b2, w3, b3); norm:
  * SQ_MLP_NORM floats; either may be NULL.  set_params resets the optimizer. */
 int sq_mlp_get_params(sq_mlp *model, float *params, float *norm);
 int sq_mlp_set_params(sq_mlp *model, const float *params, const float *norm);
-/* descriptors of the context's periodic cell, out[(N0-1)(N1-1)(N2-1)][16] */
+/* descriptors of the context's periodic cell, out[(N0-1)(N1-1)(N2-1)][32] */
 int sq_mlp_descriptors(sq_ctx *ctx, float *out);
 /* appends the context's cell (descriptors, target rho - rho_atomic) to the
  * training set; the context holds a converged sq_scf */
 int sq_mlp_add_sample(sq_mlp *model, sq_ctx *ctx);
+/* the same with every stride-th cell point only (smaller training sets) */
+int sq_mlp_add_sample_every(sq_mlp *model, sq_ctx *ctx, int stride);
 /* epochs of Adam (learning rate lr) over the whole training set; loss = the
  * final (1/2N) sum (y - t)^2 in scaled units.  The first call fixes the
  * descriptor standardization and target scale from the set. */

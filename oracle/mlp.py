@@ -3,19 +3,32 @@
 TEST INFRASTRUCTURE ONLY: imported by tests/ as the checker of the device
 kernels.  The reference has no counterpart (its SCF starts from the
 superposition of atomic densities, lsdft_readfiles.cc:1026-1040, when it
-starts at all); this restates the network the product defines:
+starts at all); this restates the network the product defines.
 
-features  f_k(x) = sum over atom images J with r = |x - X_J| < rf of
-                   exp(-((r - mu_k)/s)^2) * (cos(pi r / rf) + 1) / 2,
-                   mu_k = k rf / (K - 1), s = rf / (K - 1), at the cell points
-                   x = h i - R (the fields' grid);
-network   y = w3 . tanh(W2 tanh(W1 xhat + b1) + b2) + b3, xhat = (f - mean)/std;
+features  at a cell point x = h i - R, over the atom images J with
+          r_J = |X_J - x| < rf (rf = 8 bohr), fc(r) = (cos(pi r/rf) + 1)/2:
+          radial   f_k = sum_J exp(-((r_J - mu_k)/s)^2) fc(r_J),
+                   mu_k = k rf/15, s = rf/15, k = 0..15;
+          angular  g_a = 2^(1-zeta) sum_{J<L} (1 + lam cos t_JL)^zeta
+                         exp(-eta (r_J^2 + r_L^2)) fc(r_J) fc(r_L),
+                   t_JL the angle at x between X_J and X_L, (eta, zeta, lam)
+                   over eta in ANG_ETA, zeta in ANG_ZETA, lam in (+1, -1),
+                   a = (ie * 4 + iz) * 2 + il -- Behler's angular symmetry
+                   functions with the grid point as the centre;
+network   y = w3 . silu(W2 silu(W1 xhat + b1) + b2) + b3, xhat = (f - mean)/std,
+          silu(z) = z / (1 + exp(-z));
 loss      L = (1/2N) sum (y - t)^2;
-parameters packed as W1[H][K], b1[H], W2[H][H], b2[H], w3[H], b3.
+parameters packed as W1[H][K], b1[H], W2[H][H], b2[H], w3[H], b3 (K = 32, H = 64).
 """
 from __future__ import annotations
 
 import numpy as np
+
+K_RAD, K_ANG = 16, 16
+K = K_RAD + K_ANG
+RF = 8.0
+ANG_ETA = (0.005, 0.02)
+ANG_ZETA = (1.0, 2.0, 4.0, 16.0)
 
 
 def n_params(K, H):
@@ -38,7 +51,7 @@ def pack(W1, b1, W2, b2, w3, b3):
     return np.concatenate([W1.ravel(), b1, W2.ravel(), b2, w3, [b3]])
 
 
-def atom_images(pos, R, rf):
+def atom_images(pos, R, rf=RF):
     """Every periodic image (shifts 2R) of the atoms that can lie within rf of
     the cell, the positions first wrapped into [-R, R)."""
     pos = np.asarray(pos, np.float64).reshape(-1, 3)
@@ -54,44 +67,75 @@ def atom_images(pos, R, rf):
     return np.array(out)
 
 
-def features(pos, h, R, G, K=16, rf=6.0):
+def point_features(x, imgs, rf=RF):
+    """The K descriptors at one point x from the image positions imgs."""
+    d = imgs - x
+    r = np.sqrt(np.sum(d * d, 1))
+    m = r < rf
+    d, r = d[m], r[m]
+    fc = 0.5 * (np.cos(np.pi * r / rf) + 1.0)
+    mu = np.arange(K_RAD) * rf / (K_RAD - 1)
+    s = rf / (K_RAD - 1)
+    f = np.zeros(K)
+    f[:K_RAD] = np.sum(np.exp(-(((r[:, None] - mu[None, :]) / s) ** 2)) * fc[:, None], 0)
+    iu, ju = np.triu_indices(len(r), 1)
+    if len(iu):
+        c = np.sum(d[iu] * d[ju], 1) / (r[iu] * r[ju])
+        rr = r[iu] * r[iu] + r[ju] * r[ju]
+        w = fc[iu] * fc[ju]
+        q = K_RAD
+        for eta in ANG_ETA:
+            e = np.exp(-eta * rr) * w
+            for zeta in ANG_ZETA:
+                for lam in (1.0, -1.0):
+                    f[q] = np.sum(2.0 ** (1.0 - zeta) * (1.0 + lam * c) ** zeta * e)
+                    q += 1
+    return f
+
+
+def features(pos, h, R, G, rf=RF):
     """[G2*G1*G0, K] descriptors at the cell points (i fastest)."""
     imgs = atom_images(pos, R, rf)
     h = np.asarray(h, np.float64)
     R = np.asarray(R, np.float64)
-    zz, yy, xx = np.meshgrid(*[h[d] * np.arange(G[d]) - R[d] for d in (2, 1, 0)], indexing="ij")
-    pts = np.stack([xx.ravel(), yy.ravel(), zz.ravel()], 1)
-    mu = np.arange(K) * rf / (K - 1)
-    s = rf / (K - 1)
-    f = np.zeros((pts.shape[0], K))
-    for X in imgs:
-        r = np.sqrt(np.sum((pts - X) ** 2, 1))
-        m = r < rf
-        if not np.any(m):
-            continue
-        rc = r[m]
-        fc = 0.5 * (np.cos(np.pi * rc / rf) + 1.0)
-        f[m] += np.exp(-(((rc[:, None] - mu[None, :]) / s) ** 2)) * fc[:, None]
-    return f
+    out = np.zeros((G[0] * G[1] * G[2], K))
+    q = 0
+    for k in range(G[2]):
+        for j in range(G[1]):
+            for i in range(G[0]):
+                out[q] = point_features(np.array([h[0] * i - R[0], h[1] * j - R[1], h[2] * k - R[2]]), imgs, rf)
+                q += 1
+    return out
+
+
+def silu(z):
+    return z / (1.0 + np.exp(-z))
+
+
+def dsilu(z):
+    s = 1.0 / (1.0 + np.exp(-z))
+    return s * (1.0 + z * (1.0 - s))
 
 
 def forward(p, xhat, K, H):
     W1, b1, W2, b2, w3, b3 = unpack(p, K, H)
-    h1 = np.tanh(xhat @ W1.T + b1)
-    h2 = np.tanh(h1 @ W2.T + b2)
-    return h2 @ w3 + b3, h1, h2
+    z1 = xhat @ W1.T + b1
+    h1 = silu(z1)
+    z2 = h1 @ W2.T + b2
+    h2 = silu(z2)
+    return h2 @ w3 + b3, h1, h2, z1, z2
 
 
 def loss_grad(p, xhat, t, K, H):
     """(L, dL/dp) for L = (1/2N) sum (y - t)^2."""
     W1, b1, W2, b2, w3, b3 = unpack(p, K, H)
-    y, h1, h2 = forward(p, xhat, K, H)
+    y, h1, h2, z1, z2 = forward(p, xhat, K, H)
     N = len(t)
     e = y - t
     L = 0.5 * np.sum(e * e) / N
     d3 = e / N
-    d2 = (d3[:, None] * w3[None, :]) * (1.0 - h2 * h2)
-    d1 = (d2 @ W2) * (1.0 - h1 * h1)
+    d2 = (d3[:, None] * w3[None, :]) * dsilu(z2)
+    d1 = (d2 @ W2) * dsilu(z1)
     g = pack(d1.T @ xhat, d1.sum(0), d2.T @ h1, d2.sum(0), h2.T @ d3, d3.sum())
     return L, g
 

@@ -27,7 +27,8 @@ EXPORTS = ["sq_last_error", "sq_default_options", "sq_create_from_deck", "sq_des
            "sq_apply_h", "sq_apply_nonlocal", "sq_tridiag_nodes_weights", "sq_density_fermi_energy", "sq_synchronize",
            "sq_get_image", "sq_default_scf_options", "sq_scf", "sq_get_cell_field", "sq_set_veff_cell", "sq_xc",
            "sq_poisson", "sq_set_early_exit", "sq_get_rule_sizes", "sq_forces", "sq_mlp_create", "sq_mlp_destroy",
-           "sq_mlp_get_params", "sq_mlp_set_params", "sq_mlp_descriptors", "sq_mlp_add_sample", "sq_mlp_train",
+           "sq_mlp_get_params", "sq_mlp_set_params", "sq_mlp_descriptors", "sq_mlp_add_sample", "sq_mlp_add_sample_every",
+           "sq_mlp_train",
            "sq_mlp_forward", "sq_mlp_loss_grad", "sq_mlp_guess", "sq_kerker"]
 
 FIELDS = {"rho": 0, "rho_out": 1, "phi": 2, "vxc": 3, "veff": 4, "b": 5, "rho_atomic": 6}
@@ -130,6 +131,7 @@ def lib():
     L.sq_mlp_set_params.argtypes = [vp, fp, fp]
     L.sq_mlp_descriptors.argtypes = [vp, fp]
     L.sq_mlp_add_sample.argtypes = [vp, vp]
+    L.sq_mlp_add_sample_every.argtypes = [vp, vp, C.c_int]
     L.sq_mlp_train.argtypes = [vp, C.c_int, C.c_double, dp]
     L.sq_mlp_forward.argtypes = [vp, fp, lg, C.c_int, C.c_int, fp]
     L.sq_mlp_loss_grad.argtypes = [vp, fp, fp, lg, dp, dp]
@@ -314,7 +316,7 @@ class Context:
         return F.reshape(-1, 3), Fl.reshape(-1, 3), Fn.reshape(-1, 3), info
 
     def descriptors(self):
-        """Initial-density network descriptors of the cell, [points, 16] (i fastest)."""
+        """Initial-density network descriptors of the cell, [points, 32] (i fastest)."""
         out = np.zeros((int(np.prod(self.cell_shape())), Mlp.K), np.float32)
         _check(lib().sq_mlp_descriptors(self._h, _fp(out)))
         return out
@@ -393,7 +395,7 @@ def _fp(a):
 class Mlp:
     """The initial-density network (sq_mlp_*): descriptors -> correction to
     the superposed atomic density, trained on converged SQ densities."""
-    K, H, PARAMS, NORM = 16, 64, 5313, 33
+    K, H, PARAMS, NORM = 32, 64, 6337, 65
 
     def __init__(self, device=0, seed=0):
         self._h = C.c_void_p()
@@ -427,8 +429,8 @@ class Mlp:
         n = None if norm is None else np.ascontiguousarray(norm, np.float32)
         _check(lib().sq_mlp_set_params(self._h, _fp(p), _fp(n)))
 
-    def add_sample(self, ctx):
-        _check(lib().sq_mlp_add_sample(self._h, ctx._h))
+    def add_sample(self, ctx, stride=1):
+        _check(lib().sq_mlp_add_sample_every(self._h, ctx._h, int(stride)))
 
     def train(self, epochs, lr=3e-3):
         loss = C.c_double()

@@ -2054,7 +2054,7 @@ struct sq_mlp {
     int device = 0;
     hipStream_t stream = nullptr;
     int K = kMlpK, H = kMlpH, np = 0;
-    double rf = 6.0;
+    double rf = kMlpRf;
     float *p = nullptr, *m = nullptr, *v = nullptr;
     double *g = nullptr;
     int t = 0;
@@ -2109,11 +2109,15 @@ int mlp_descriptors_dev(sq_ctx *c, double rf, float *dout)
     std::vector<double4> img = mlp_images(S, rf);
     DevBufs B;
     double4 *dimg;
-    if ((rc = B.get(&dimg, img.size()))) return rc;
+    int *dover, over = 0;
+    if ((rc = B.get(&dimg, img.size())) || (rc = B.get(&dover, 1))) return rc;
     HIP_TRY(hipMemcpy(dimg, img.data(), img.size() * sizeof(double4), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemset(dover, 0, sizeof(int)));
     const double h[3] = {S.h[0], S.h[1], S.h[2]}, R[3] = {S.range[0], S.range[1], S.range[2]};
-    HIP_TRY(launch_mlp_features(dimg, (int)img.size(), G, h, R, kMlpK, rf, dout, c->stream));
+    HIP_TRY(launch_mlp_features(dimg, (int)img.size(), G, h, R, kMlpK, rf, dout, dover, c->stream));
+    HIP_TRY(hipMemcpyAsync(&over, dover, sizeof(int), hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
+    if (over) return set_err(SQ_ERR_RANGE, "descriptors: more atoms within the cutoff of a point than the kernel holds");
     return SQ_OK;
 }
 
@@ -2210,14 +2214,16 @@ int sq_mlp_descriptors(sq_ctx *c, float *out)
     const long ng = (long)G[0] * G[1] * G[2];
     DevBufs B;
     float *d;
-    if ((rc = B.get(&d, ng * kMlpK)) || (rc = mlp_descriptors_dev(c, 6.0, d))) return rc;
+    if ((rc = B.get(&d, ng * kMlpK)) || (rc = mlp_descriptors_dev(c, kMlpRf, d))) return rc;
     HIP_TRY(hipMemcpy(out, d, ng * kMlpK * sizeof(float), hipMemcpyDeviceToHost));
     return SQ_OK;
 }
 
-int sq_mlp_add_sample(sq_mlp *m, sq_ctx *c)
+int sq_mlp_add_sample(sq_mlp *m, sq_ctx *c) { return sq_mlp_add_sample_every(m, c, 1); }
+
+int sq_mlp_add_sample_every(sq_mlp *m, sq_ctx *c, int stride)
 {
-    if (!m || !c) return set_err(SQ_ERR_ARG, "null argument");
+    if (!m || !c || stride < 1) return set_err(SQ_ERR_ARG, "bad argument");
     if (!c->stream) return set_err(SQ_ERR_STATE, "host-only context (created with device < 0)");
     if (!c->scf.force_ready) return set_err(SQ_ERR_STATE, "no SCF density on this context (call sq_scf first)");
     const System &S = c->sys;
@@ -2231,21 +2237,25 @@ int sq_mlp_add_sample(sq_mlp *m, sq_ctx *c)
     if ((rc = sq_get_cell_field(c, SQ_FIELD_RHO, rho.data()))) return rc;
     Status st = build_atomic_density(S, rat, sc);
     if (!st.ok()) return set_err(st.code, st.msg);
-    m->X.insert(m->X.end(), f.begin(), f.end());
-    for (long q = 0; q < ng; q++) m->T.push_back((float)(rho[q] - rat[q]));
+    for (long q = 0; q < ng; q += stride) {
+        m->X.insert(m->X.end(), f.begin() + q * kMlpK, f.begin() + (q + 1) * kMlpK);
+        m->T.push_back((float)(rho[q] - rat[q]));
+    }
     return SQ_OK;
 }
 
 // Activations, deltas and partial sums of one pass over N rows.
 struct MlpScratch {
-    float *H1 = nullptr, *H2 = nullptr, *Y = nullptr, *D1 = nullptr, *D2 = nullptr, *D3 = nullptr;
+    float *H1 = nullptr, *H2 = nullptr, *Z1 = nullptr, *Z2 = nullptr, *Y = nullptr, *D1 = nullptr, *D2 = nullptr,
+          *D3 = nullptr;
     double *part = nullptr, *lsum = nullptr;
     int nchunk = 1;
     int alloc(DevBufs &B, long N, int H)
     {
         nchunk = (int)std::max<long>(1, std::min<long>(128, N / 256));
         int rc;
-        if ((rc = B.get(&H1, N * H)) || (rc = B.get(&H2, N * H)) || (rc = B.get(&Y, N)) || (rc = B.get(&D1, N * H)) ||
+        if ((rc = B.get(&H1, N * H)) || (rc = B.get(&H2, N * H)) || (rc = B.get(&Z1, N * H)) || (rc = B.get(&Z2, N * H)) ||
+            (rc = B.get(&Y, N)) || (rc = B.get(&D1, N * H)) ||
             (rc = B.get(&D2, N * H)) || (rc = B.get(&D3, N)) || (rc = B.get(&part, (size_t)nchunk * H * H)) ||
             (rc = B.get(&lsum, 1)))
             return rc;
@@ -2261,8 +2271,8 @@ static int mlp_loss_grad_dev(sq_mlp *m, const float *X, const float *T, long N, 
     float *H1 = W.H1, *H2 = W.H2, *Y = W.Y, *D1 = W.D1, *D2 = W.D2, *D3 = W.D3;
     double *part = W.part, *lsum = W.lsum;
     const int nchunk = W.nchunk;
-    HIP_TRY(launch_mlp_forward(X, N, m->p, nullptr, nullptr, H1, H2, Y, 0, m->stream));
-    HIP_TRY(launch_mlp_deltas(H1, H2, Y, T, N, m->p, D1, D2, D3, m->stream));
+    HIP_TRY(launch_mlp_forward(X, N, m->p, nullptr, nullptr, H1, H2, W.Z1, W.Z2, Y, 0, m->stream));
+    HIP_TRY(launch_mlp_deltas(W.Z1, W.Z2, Y, T, N, m->p, D1, D2, D3, m->stream));
     double *g = m->g;
     HIP_TRY(launch_outer_sum(D1, H, X, K, N, part, nchunk, g, m->stream));                          // W1
     HIP_TRY(launch_outer_sum(D1, H, nullptr, 1, N, part, nchunk, g + H * K, m->stream));            // b1
@@ -2309,8 +2319,8 @@ int sq_mlp_forward(sq_mlp *m, const float *X, long N, int bf16, int normalize, f
     HIP_TRY(hipMemcpy(dX, X, N * m->K * sizeof(float), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(dmean, m->mean.data(), m->K * sizeof(float), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(drstd, m->rstd.data(), m->K * sizeof(float), hipMemcpyHostToDevice));
-    HIP_TRY(launch_mlp_forward(dX, N, m->p, normalize ? dmean : nullptr, normalize ? drstd : nullptr, nullptr, nullptr, dY,
-                               bf16, m->stream));
+    HIP_TRY(launch_mlp_forward(dX, N, m->p, normalize ? dmean : nullptr, normalize ? drstd : nullptr, nullptr, nullptr,
+                               nullptr, nullptr, dY, bf16, m->stream));
     HIP_TRY(hipMemcpyAsync(Y, dY, N * sizeof(float), hipMemcpyDeviceToHost, m->stream));
     HIP_TRY(hipStreamSynchronize(m->stream));
     return SQ_OK;

@@ -125,13 +125,16 @@ struct ForceParams {
 hipError_t launch_nl_force(const ForceParams &F, hipStream_t st);
 
 // ---- initial-density network (sq_mlp.hip): K descriptors, H hidden units
-constexpr int kMlpK = 16, kMlpH = 64;
+constexpr int kMlpK = 32, kMlpH = 64;
+constexpr double kMlpRf = 8.0;   // descriptor cutoff, bohr
 int mlp_param_count(int K, int H);
+// overflow: device int counting points with more neighbours than the kernel holds
 hipError_t launch_mlp_features(const double4 *img, int nimg, const int G[3], const double h[3], const double R[3], int K,
-                               double rf, float *out, hipStream_t st);
+                               double rf, float *out, int *overflow, hipStream_t st);
+// Z1, Z2 (may be null): pre-activations of the two hidden layers, for the deltas
 hipError_t launch_mlp_forward(const float *X, long N, const float *P, const float *mean, const float *rstd, float *H1,
-                              float *H2, float *Y, int bf16, hipStream_t st);
-hipError_t launch_mlp_deltas(const float *H1, const float *H2, const float *Y, const float *T, long N, const float *P,
+                              float *H2, float *Z1, float *Z2, float *Y, int bf16, hipStream_t st);
+hipError_t launch_mlp_deltas(const float *Z1, const float *Z2, const float *Y, const float *T, long N, const float *P,
                              float *D1, float *D2, float *D3, hipStream_t st);
 // out[i][j] = sum_n A[n][i] B[n][j] (B == nullptr: 1), in nchunk double partials summed in order
 hipError_t launch_outer_sum(const float *A, int a, const float *B, int b, long N, double *partial, int nchunk, double *out,

@@ -19,7 +19,7 @@ from conftest import MG1, write_variant
 import sqminiapp_amd as sq
 
 pytestmark = pytest.mark.gpu
-K, H = 16, 64
+K, H = 32, 64
 
 
 def _cell(tmp, name, seed, sigma=0.15):
@@ -34,8 +34,8 @@ def _bf16_forward(p, x):
     W1, b1, W2, b2, w3, b3 = [np.asarray(M.bf16(np.asarray(a, np.float32)), np.float64) if np.ndim(a) else
                               float(M.bf16(np.float32([a]))[0]) for a in M.unpack(p, K, H)]
     xh = M.bf16(x.astype(np.float32)).astype(np.float64)
-    h1 = M.bf16(np.tanh(xh @ W1.T + b1).astype(np.float32)).astype(np.float64)
-    h2 = M.bf16(np.tanh(h1 @ W2.T + b2).astype(np.float32)).astype(np.float64)
+    h1 = M.bf16(M.silu(xh @ W1.T + b1).astype(np.float32)).astype(np.float64)
+    h2 = M.bf16(M.silu(h1 @ W2.T + b2).astype(np.float32)).astype(np.float64)
     return h2 @ w3 + b3
 
 
