@@ -62,7 +62,7 @@ struct sq_ctx {
     bool zbatch = false;
     int zb_cap[5] = {}, zb_grid[2][5] = {};   // grid per precision and B
     size_t zb_lds[5] = {};
-    int zb_off[5][10] = {};
+    int zb_off[5][11] = {};
     // the configuration the autotuner chose per precision: nodes per workgroup
     // (1: the one-node z-march kernel in its zwg shape) and gather trip
     int cfg_B[2] = {1, 1}, cfg_ug[2] = {0, 0};
@@ -482,6 +482,7 @@ int create_device(sq_ctx *c, int device)
         off[7] = (int)z; z = align16(z + (size_t)B * 2 * (S.nqp + 1) * sizeof(double));   // early-exit rule
         off[8] = (int)z; z = align16(z + 2 * (size_t)(S.nloc[0] + 12) * (S.nloc[1] + 12) * 16);   // plane (16 B a point)
         off[9] = (int)z; z = align16(z + (size_t)cap * sizeof(int4));      // atom runs
+        off[10] = (int)z; z = align16(z + (size_t)cap * B * sizeof(double));   // per-atom shares
         c->zb_cap[B] = cap;
         c->zb_lds[B] = cap <= 256 && z <= 150 * 1024 ? z : 0;
     }
@@ -615,6 +616,7 @@ KParams make_batch_params(const sq_ctx *c, int mode, int precision, int B, const
     P.lds_local = off[7];
     P.lds_plane = off[8];
     P.lds_run = off[9];
+    P.lds_share = off[10];
     P.basis = precision == SQ_FP64 ? (const void *)c->bas64 : (const void *)c->bas32;
     P.img_cap = c->zb_cap[B];
     P.batch = batch;

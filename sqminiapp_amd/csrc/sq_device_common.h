@@ -25,7 +25,8 @@ __device__ __forceinline__ T wave_sum(T v)
     return v;
 }
 
-// Compensated (TwoSum) pair used by the FP32 form's dot products.
+// Compensated pair used by the FP32 form's dot products: TwoSum for the
+// additions, TwoProduct (fma) for the products (Ogita-Rump-Oishi Dot2).
 struct CSum {
     float hi, lo;
 };
@@ -40,6 +41,16 @@ __device__ __forceinline__ CSum csum_add(CSum x, float y)
 {
     CSum t = two_sum(x.hi, y);
     t.lo += x.lo;
+    return t;
+}
+// Dot2 accumulation of a product: its rounding error (fma) joins the
+// compensation term with the TwoSum error of the addition
+__device__ __forceinline__ CSum csum_add_prod(CSum x, float a, float b)
+{
+    const float p = a * b;
+    const float e = fmaf(a, b, -p);
+    CSum t = two_sum(x.hi, p);
+    t.lo += x.lo + e;
     return t;
 }
 __device__ __forceinline__ CSum csum_merge(CSum x, CSum y)

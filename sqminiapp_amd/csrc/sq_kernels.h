@@ -68,7 +68,7 @@ struct KParams {
     // each {id of its lowest node, nb}
     const int2 *batch;
     const void *basis;           // (t0, t2, t2 z, t2 z^2) per image box point (element type = precision)
-    int lds_run;                 // batched kernel: per-image atom run and type
+    int lds_run, lds_share;      // batched kernel: per-image atom run and type; per-atom shares of <v, V_nl v>
 };
 
 hipError_t launch_lanczos(const KParams &P, int precision, int grid, size_t lds, hipStream_t st);

@@ -121,7 +121,7 @@ __global__ void __launch_bounds__(kWG) lanczos_kernel(KParams P)
                 Bu[p] = u;
                 const T vc = sv * Bv[p];
                 if constexpr (sizeof(T) == 8) pa += vc * u;
-                else ca = csum_add(ca, vc * u);
+                else ca = csum_add_prod(ca, vc, u);
             }
             double a;
             if constexpr (sizeof(T) == 8) a = block_sum(pa, red);
@@ -142,7 +142,7 @@ __global__ void __launch_bounds__(kWG) lanczos_kernel(KParams P)
                 } else {
                     w = fmaf(ta, vc, u);
                     w = fmaf(tb, vpc, w);
-                    cb = csum_add(cb, w * w);
+                    cb = csum_add_prod(cb, w, w);
                 }
                 Bvp[p] = w;
             }

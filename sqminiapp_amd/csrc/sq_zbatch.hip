@@ -46,17 +46,6 @@ __device__ __forceinline__ void vset(V &v, int b, T x) { v[b] = x; }
 
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-// FP32 Dot2 accumulation: the product's rounding error (fma) joins the
-// compensation term of the TwoSum pair.
-__device__ __forceinline__ CSum csum_add_prod(CSum x, float a, float b)
-{
-    const float p = a * b;
-    const float e = fmaf(a, b, -p);
-    CSum t = two_sum(x.hi, p);
-    t.lo += x.lo + e;
-    return t;
-}
-
 }  // namespace
 
 // Sum of B values over the workgroup (fixed tree, deterministic); every
@@ -127,6 +116,7 @@ __global__ void __launch_bounds__(1024) lanczos_zbatch(KParams P)
     double *s_ab = reinterpret_cast<double *>(smem + P.lds_local);   // [B][2][nqp+1] (early exit)
     VB *plane = reinterpret_cast<VB *>(smem + P.lds_plane);
     int4 *trun = reinterpret_cast<int4 *>(smem + P.lds_run);          // [img]: first, last image of its atom; type
+    double *pshare = reinterpret_cast<double *>(smem + P.lds_share);  // [img][B]: an atom's share of <v, V_nl v>
     __shared__ int s_count;
     __shared__ T s_w[3][8];
     __shared__ long s_next;
@@ -399,40 +389,48 @@ __global__ void __launch_bounds__(1024) lanczos_zbatch(KParams P)
                 }
             }
             __syncthreads();
-            // ---- alpha per node and image from the A of the images of the same atom
+            // ---- per atom (the first list entry of its run of images) and node:
+            // alpha of the atom's images and the atom's share of <v_b, V_nl v_b>;
+            // then per node the shares in ascending atom order.  Images or atoms
+            // outside a node's cube add exact zeros, so neither sum depends on the
+            // batch the node runs in.
             for (int e = tid; e < nimg * B; e += WG) {
                 const int li = e / B, b = e - li * B;
                 const int4 run = trun[li];
+                if (run.x != li) continue;
                 const int s = run.x, t = run.y;
                 const int nlm = tdim[li].w & 255;
                 const double *gam = P.gamma_ch + run.z * kMaxChannels;
-                if (P.mode == 0) {
-                    double al = 0.0;
+                double share = 0.0;
+                if (P.mode == 0) {   // one coefficient per atom
+                    double al = 0.0, sa = 0.0;
                     for (int q = s; q <= t; q++)
-                        for (int ch = 0; ch < nlm; ch++) al += gam[ch] * A[(q * B + b) * NLM + ch];
-                    for (int ch = 0; ch < NLM; ch++) alpha[(li * B + b) * NLM + ch] = al;
-                } else {
-                    for (int ch = 0; ch < NLM; ch++) {
-                        double al = 0.0;
-                        if (ch < nlm) {
-                            for (int q = s; q <= t; q++) al += A[(q * B + b) * NLM + ch];
-                            al = gam[ch] * al;
+                        for (int ch = 0; ch < nlm; ch++) {
+                            const double x = A[(q * B + b) * NLM + ch];
+                            al += gam[ch] * x;
+                            sa += x;
                         }
-                        alpha[(li * B + b) * NLM + ch] = al;
+                    for (int q = s; q <= t; q++)
+                        for (int ch = 0; ch < NLM; ch++) alpha[(q * B + b) * NLM + ch] = al;
+                    share = al * sa;
+                } else {             // per (l,m) channel
+                    for (int ch = 0; ch < NLM; ch++) {
+                        double sa = 0.0, al = 0.0;
+                        if (ch < nlm) {
+                            for (int q = s; q <= t; q++) sa += A[(q * B + b) * NLM + ch];
+                            al = gam[ch] * sa;
+                            share = fma(al, sa, share);
+                        }
+                        for (int q = s; q <= t; q++) alpha[(q * B + b) * NLM + ch] = al;
                     }
                 }
+                pshare[li * B + b] = share;
             }
             __syncthreads();
-            // ---- nonlocal part of <v_b, H v_b> from the gathered sums, one thread
-            // per node in ascending image order (images outside a node's cube
-            // add exact zeros, so the sum does not depend on the batch)
             if (tid < B) {
                 double sn = 0.0;
-                for (int li = 0; li < nimg; li++) {
-                    const int nlm = tdim[li].w & 255;
-                    const double *al = alpha + (li * B + tid) * NLM, *ga = A + (li * B + tid) * NLM;
-                    for (int ch = 0; ch < nlm; ch++) sn = fma(al[ch], ga[ch], sn);
-                }
+                for (int li = 0; li < nimg; li++)
+                    if (trun[li].x == li) sn += pshare[li * B + tid];
                 red[(B + 1) * NW + tid] = sn;
             }
             __syncthreads();

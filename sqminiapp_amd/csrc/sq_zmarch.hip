@@ -427,7 +427,7 @@ lanczos_zmarch(KParams P)
                             const T u = fma(dg[k], vc, sx) + (sy + sz);
                             Bu[col[k] + z * ncol] = u;
                             if constexpr (sizeof(T) == 8) pa = fma(vc, u, pa);
-                            else ca = csum_add(ca, vc * u);
+                            else ca = csum_add_prod(ca, vc, u);
                         }
                         // plane z - O leaves the window; plane z + O + 1 takes its slot
                         win[k][s % W] = next[k];
@@ -540,7 +540,7 @@ lanczos_zmarch(KParams P)
                         } else {
                             w = fmaf(ta, vc, uu[u]);
                             w = fmaf(tb, vpc, w);
-                            cb = csum_add(cb, w * w);
+                            cb = csum_add_prod(cb, w, w);
                         }
                         Bvp[p] = w;
                     }

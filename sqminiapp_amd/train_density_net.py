@@ -2,6 +2,7 @@
 128-atom Mg cells and writes its weights.
 
     python -m sqminiapp_amd.train_density_net [--c16 24] [--c128 4] [--epochs 2000] [--out FILE]
+    python -m sqminiapp_amd.train_density_net --evaluate C16,C128 [--out FILE]
 
 Training cells: the 16-atom hcp Mg cell and the 128-atom cell with its 4-site
 vacancy cluster (sqminiapp_amd.decks: h = 0.5, 29^3 cubes, 80 Lanczos steps,
@@ -81,6 +82,39 @@ def load(path=DEFAULT_OUT, device=0):
     return m
 
 
+def evaluate(cells=("C16", "C128"), path=DEFAULT_OUT, verbose=True):
+    """The chosen SCF form on held-out cells from the superposed atoms and from
+    the network's BF16 guess: iterations, wall time, initial errors, energy gap."""
+    root = tempfile.mkdtemp(prefix="sq_eval_")
+    out = []
+    for name in cells:
+        d, nm = decks.deck(name, root)
+        net = load(path)
+        if net is None:
+            raise RuntimeError(f"no network weights at {path}")
+        with Context(d, nm, cell=True) as c:
+            kw = dict(tol=1e-5, max_iter=80, kerker=1.0, mixing=0.7)
+            ra = c.scf(SQ_NL_KB, SQ_FP32, **kw)
+            rho = c.cell_field("rho")
+            atomic = c.cell_field("rho_atomic")
+            atomic = atomic * (c.info.noetot / (atomic.sum() * c.info.dV))
+            t0 = time.time()
+            g = net.guess(c, bf16=True)
+            tg = time.time() - t0
+            rl = c.scf(SQ_NL_KB, SQ_FP32, rho0=g, **kw)
+            norm = np.linalg.norm(rho)
+            row = dict(cell=name, atoms_iterations=ra["iterations"], network_iterations=rl["iterations"],
+                       atoms_wall_s=ra["wall_s"], network_wall_s=rl["wall_s"], guess_s=tg,
+                       error_atoms=float(np.linalg.norm(atomic - rho) / norm),
+                       error_network=float(np.linalg.norm(g - rho) / norm),
+                       d_energy_per_atom=abs(rl["e_total"] - ra["e_total"]) / c.info.nAtoms)
+        net.close()
+        out.append(row)
+        if verbose:
+            print(row, flush=True)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--c16", type=int, default=24)
@@ -90,7 +124,11 @@ def main():
     ap.add_argument("--epochs", type=int, default=2000)
     ap.add_argument("--lr", type=float, default=3e-3)
     ap.add_argument("--out", default=DEFAULT_OUT)
+    ap.add_argument("--evaluate", default=None, help="held-out cells to evaluate the saved weights on (no training)")
     a = ap.parse_args()
+    if a.evaluate:
+        evaluate(a.evaluate.split(","), a.out)
+        return
     path, l0, l1 = train(a.c16, a.c128, a.epochs, a.lr, a.out, stride16=a.stride16, stride128=a.stride128)
     print(f"wrote {path} (loss {l0:.4f} -> {l1:.4f})")
 
