@@ -152,6 +152,19 @@ int sq_apply_nonlocal(sq_ctx *ctx, int nonlocal_mode, int Kp, int Jp, int Ip, co
 int sq_tridiag_nodes_weights(sq_ctx *ctx, const double *diag, const double *subdiag, int n, long count,
                              double *lambda, double *w, double *lmin, double *lmax);
 
+/* sq_tridiag_nodes_weights for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc); it has no counterpart in the reference.  diag, subdiag, lambda,
+ * w [count][n] and lmin, lmax [count] are DEVICE buffers of the context's
+ * device, all required, the inputs complete before the call (it orders
+ * nothing against other streams).  The QL workspace is the library's static
+ * device array of 32 MiB per device, 3*8*n*count + 4*count bytes plus
+ * alignment (n = 80: count <= 17439); larger batches return SQ_ERR_ARG.  The
+ * call allocates nothing, runs on the context's stream and waits for it.
+ * Calls that overlap on one device race on that array. */
+int sq_tridiag_nodes_weights_noalloc(sq_ctx *ctx, const double *diag, const double *subdiag, int n, long count,
+                                     double *lambda, double *w, double *lmin, double *lmax);
+
 /* Density, Fermi level and band/entropy energy from the last sq_run, which must
  * have covered every node of the rank box.  The reference declares these only
  * (CalculateDensity_lssgq, constraint_lssgq, FermiDirac, findRootBrent,

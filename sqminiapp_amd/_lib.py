@@ -24,7 +24,7 @@ STATUS = {0: "SQ_OK", 1: "SQ_ERR_IO", 2: "SQ_ERR_FORMAT", 3: "SQ_ERR_RANGE", 4: 
 # every symbol include/sq_amd.h declares
 EXPORTS = ["sq_last_error", "sq_default_options", "sq_create_from_deck", "sq_destroy", "sq_get_info", "sq_set_veff",
            "sq_run", "sq_get_nodes_and_weights", "sq_get_lanczos", "sq_nodes_and_weights", "sq_last_timing",
-           "sq_apply_h", "sq_apply_nonlocal", "sq_tridiag_nodes_weights", "sq_density_fermi_energy", "sq_synchronize",
+           "sq_apply_h", "sq_apply_nonlocal", "sq_tridiag_nodes_weights", "sq_tridiag_nodes_weights_noalloc", "sq_density_fermi_energy", "sq_synchronize",
            "sq_get_image", "sq_default_scf_options", "sq_scf", "sq_get_cell_field", "sq_set_veff_cell", "sq_xc",
            "sq_poisson", "sq_set_early_exit", "sq_get_rule_sizes", "sq_forces", "sq_mlp_create", "sq_mlp_destroy",
            "sq_mlp_get_params", "sq_mlp_set_params", "sq_mlp_descriptors", "sq_mlp_add_sample", "sq_mlp_add_sample_every",
@@ -137,6 +137,7 @@ def lib():
     L.sq_mlp_loss_grad.argtypes = [vp, fp, fp, lg, dp, dp]
     L.sq_mlp_guess.argtypes = [vp, vp, C.c_int, dp]
     L.sq_tridiag_nodes_weights.argtypes = [vp, dp, dp, C.c_int, lg, dp, dp, dp, dp]
+    L.sq_tridiag_nodes_weights_noalloc.argtypes = [vp, vp, vp, C.c_int, lg, vp, vp, vp, vp]
     L.sq_density_fermi_energy.argtypes = [vp, C.c_double, C.c_double, dp, dp, dp, dp]
     L.sq_synchronize.argtypes = [vp]
     L.sq_get_image.argtypes = [vp, C.c_int, dp, ip, ip, ip, ip, dp]
@@ -366,6 +367,11 @@ class Context:
         _check(lib().sq_tridiag_nodes_weights(self._h, _dp(diag), _dp(sub), n, count, _dp(lam), _dp(w), _dp(lmin),
                                               _dp(lmax)))
         return lam, w, lmin, lmax
+
+    def tridiag_nodes_weights_noalloc(self, diag, subdiag, n, count, lam, w, lmin, lmax):
+        """sq_tridiag_nodes_weights_noalloc on device buffers given by address
+        (diag, subdiag, lam, w: count*n doubles; lmin, lmax: count doubles)."""
+        _check(lib().sq_tridiag_nodes_weights_noalloc(self._h, diag, subdiag, int(n), int(count), lam, w, lmin, lmax))
 
     def density_fermi_energy(self, nelec=0.0, beta=0.0, want_rho=True):
         lf, eb, mts = C.c_double(), C.c_double(), C.c_double()

@@ -330,6 +330,8 @@ int create_device(sq_ctx *c, int device)
     c->n_cu = prop.multiProcessorCount;
     HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     for (auto &e : c->ev) HIP_TRY(hipEventCreate(&e));
+    // the static workspace of sq_tridiag_nodes_weights_noalloc on this device
+    if (noalloc_scratch_init(device) != hipSuccess) return set_err(SQ_ERR_DEVICE, "static workspace symbol not found");
     // projector tables
     const int M = (int)S.images.size();
     std::vector<int4> lo(M), dim(M);
@@ -1125,6 +1127,62 @@ int sq_tridiag_nodes_weights(sq_ctx *c, const double *diag, const double *subdia
     (void)hipFree(buf);
     (void)hipFree(st);
     return rc;
+}
+
+// A caller's device buffer: device memory of the context's device, with at
+// least `bytes` from p to the end of its allocation.
+static int check_device_buffer(const sq_ctx *c, const void *p, size_t bytes, const char *name)
+{
+    hipPointerAttribute_t at;
+    if (hipPointerGetAttributes(&at, p) != hipSuccess || at.type != hipMemoryTypeDevice || at.device != c->device) {
+        (void)hipGetLastError();
+        return set_err(SQ_ERR_ARG, std::string(name) + ": not device memory of the context's device");
+    }
+    hipDeviceptr_t base = nullptr;
+    size_t size = 0;
+    if (hipMemGetAddressRange(&base, &size, (hipDeviceptr_t)p) != hipSuccess) {
+        (void)hipGetLastError();
+        return set_err(SQ_ERR_ARG, std::string(name) + ": allocation range unknown");
+    }
+    if ((const char *)p + bytes > (const char *)base + size)
+        return set_err(SQ_ERR_ARG, std::string(name) + ": smaller than the tridiagonal batch");
+    return SQ_OK;
+}
+
+int sq_tridiag_nodes_weights_noalloc(sq_ctx *c, const double *diag, const double *subdiag, int n, long count,
+                                     double *lambda, double *w, double *lmin, double *lmax)
+{
+    if (!c || !diag || !subdiag || !lambda || !w || !lmin || !lmax) return set_err(SQ_ERR_ARG, "null argument");
+    if (!c->stream) return set_err(SQ_ERR_STATE, "host-only context (created with device < 0)");
+    if (n < 1 || n > kMaxNqp || count <= 0) return set_err(SQ_ERR_ARG, "bad tridiagonal size");
+    const size_t need = noalloc_ql_bytes(n, count);
+    if (need > kNoAllocScratchBytes) {
+        long lo = 0, hi = (long)(kNoAllocScratchBytes / sizeof(double));
+        while (lo < hi) {
+            const long mid = (lo + hi + 1) / 2;
+            if (noalloc_ql_bytes(n, mid) <= kNoAllocScratchBytes) lo = mid;
+            else hi = mid - 1;
+        }
+        return set_err(SQ_ERR_ARG, "tridiagonal batch needs " + std::to_string(need) + " bytes of workspace; the "
+                       "static workspace holds " + std::to_string(kNoAllocScratchBytes) + ": at n = " +
+                       std::to_string(n) + ", count <= " + std::to_string(lo));
+    }
+    HIP_TRY(hipSetDevice(c->device));
+    void *ws = noalloc_scratch(c->device);
+    if (!ws) return set_err(SQ_ERR_STATE, "static workspace address not resolved on this device");
+    const size_t m = (size_t)n * count * sizeof(double);
+    int rc;
+    if ((rc = check_device_buffer(c, diag, m, "diag")) || (rc = check_device_buffer(c, subdiag, m, "subdiag")) ||
+        (rc = check_device_buffer(c, lambda, m, "lambda")) || (rc = check_device_buffer(c, w, m, "w")) ||
+        (rc = check_device_buffer(c, lmin, count * sizeof(double), "lmin")) ||
+        (rc = check_device_buffer(c, lmax, count * sizeof(double), "lmax")))
+        return rc;
+    long long *dbad = nullptr, bad = -1;
+    HIP_TRY(launch_ql_noalloc(ws, diag, subdiag, n, count, lambda, w, lmin, lmax, &dbad, c->stream));
+    HIP_TRY(hipMemcpyAsync(&bad, dbad, sizeof bad, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    if (bad >= 0) return set_err(SQ_ERR_QL, "QL: more than 50 sweeps for tridiagonal " + std::to_string(bad));
+    return SQ_OK;
 }
 
 // Fermi level by bisection on the device electron count, then rho (into the

@@ -84,6 +84,17 @@ hipError_t launch_apply(const KParams &P, long id, const double *x, double *y, i
                         hipStream_t st);
 hipError_t launch_ql(const double *a, const double *b, int n, int stride, long count, double *lam, double *w,
                      double *lmin, double *lmax, int *status, double *ws, hipStream_t st, const int *nsteps = nullptr);
+// QL of host-supplied device tridiagonals with its workspace in the library's
+// static device array (no allocation): the array's per-device address
+// (noalloc_scratch_init, with the device current), the bytes a call needs, and
+// the launch, which returns the device address of the first-failure index.
+constexpr size_t kNoAllocScratchBytes = (size_t)32 << 20;
+constexpr int kNoAllocMaxDevices = 64;
+hipError_t noalloc_scratch_init(int device);
+void *noalloc_scratch(int device);
+size_t noalloc_ql_bytes(int n, long count);
+hipError_t launch_ql_noalloc(void *ws, const double *a, const double *b, int n, long count, double *lam, double *w,
+                             double *lmin, double *lmax, long long **first_bad, hipStream_t st);
 hipError_t launch_occupation(const double *lam, const double *w, long nq, double beta, double lf, double *partial,
                              int nblocks, double *out, hipStream_t st);
 hipError_t launch_density_energy(const double *lam, const double *w, long n_nodes, int nqp, double beta, double lf,

@@ -16,6 +16,9 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <climits>
+#include <mutex>
+
 #include "sq_device_common.h"
 
 namespace sqamd {
@@ -481,6 +484,83 @@ hipError_t launch_density_energy_dev(const double *lam, const double *w, long n_
 hipError_t launch_lam_range(const double *lmin, const double *lmax, long n, double *out, hipStream_t st)
 {
     hipLaunchKernelGGL(lam_range, dim3(1), dim3(kRedThreads), 0, st, lmin, lmax, n, out);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------- QL without allocation
+// Workspace of sq_tridiag_nodes_weights_noalloc: a static device array of
+// kNoAllocScratchBytes, one copy per GPU at its own address, held until the
+// process ends.  In host code `scratch` names a host-side stand-in, never the
+// device memory: its address on each device comes from hipGetSymbolAddress
+// (noalloc_scratch_init, once per device, with that device current) and only
+// that address is handed to kernels.
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace {
+std::mutex g_scratch_mu;
+void *g_scratch_dev[kNoAllocMaxDevices];
+
+size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+}  // namespace
+
+hipError_t noalloc_scratch_init(int device)
+{
+    if (device < 0 || device >= kNoAllocMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_dev[device]) return hipSuccess;
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    if (!p) return hipErrorInvalidSymbol;
+    g_scratch_dev[device] = p;
+    return hipSuccess;
+}
+
+void *noalloc_scratch(int device)
+{
+    if (device < 0 || device >= kNoAllocMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    return g_scratch_dev[device];
+}
+
+// workspace layout: QL d, e and z row 0 [3][n][count] doubles | status[count] |
+// the index of the first failed tridiagonal (long long)
+size_t noalloc_ql_bytes(int n, long count)
+{
+    return align256((size_t)3 * n * count * sizeof(double)) + align256((size_t)count * sizeof(int)) + 256;
+}
+
+// Index of the first tridiagonal whose QL did not converge, -1 when none
+// (one workgroup; each thread scans its stride in ascending order).
+__global__ void __launch_bounds__(256) first_failure(const int *status, long count, long long *out)
+{
+    __shared__ long long m[256];
+    long long best = -1;
+    for (long i = threadIdx.x; i < count; i += 256)
+        if (status[i]) { best = i; break; }
+    m[threadIdx.x] = best < 0 ? LLONG_MAX : best;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) m[threadIdx.x] = min(m[threadIdx.x], m[threadIdx.x + s]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = m[0] == LLONG_MAX ? -1 : m[0];
+}
+
+hipError_t launch_ql_noalloc(void *ws, const double *a, const double *b, int n, long count, double *lam, double *w,
+                             double *lmin, double *lmax, long long **first_bad, hipStream_t st)
+{
+    if (!ws || n < 1 || n > kMaxNqp || count <= 0 || noalloc_ql_bytes(n, count) > kNoAllocScratchBytes)
+        return hipErrorInvalidValue;
+    unsigned char *base = static_cast<unsigned char *>(ws);
+    double *d = reinterpret_cast<double *>(base);
+    int *status = reinterpret_cast<int *>(base + align256((size_t)3 * n * count * sizeof(double)));
+    long long *bad = reinterpret_cast<long long *>(reinterpret_cast<unsigned char *>(status) +
+                                                   align256((size_t)count * sizeof(int)));
+    hipError_t e = launch_ql(a, b, n, n, count, lam, w, lmin, lmax, status, d, st);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(first_failure, dim3(1), dim3(256), 0, st, status, count, bad);
+    *first_bad = bad;
     return hipGetLastError();
 }
 
