@@ -249,6 +249,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--fp32-steps", type=int, default=None)
     ap.add_argument("--no-scf", action="store_true", help="skip the SCF wall-time measurement")
+    ap.add_argument("--no-factors", action="store_true", help="skip the kernel-factor passes (profiling runs)")
     ap.add_argument("--scf-iter", type=int, default=60)
     ap.add_argument("--scf-config", default=None, help="deck of the SCF measurement (default: --config)")
     ap.add_argument("--scf-verbose", action="store_true", help="one line per SCF iteration on stderr")
@@ -314,15 +315,16 @@ def main():
                      "mfma": mfma}}
     # the kernel factors, same slabs: FP32 in the one-node kernel (precision alone), then batched
     ctx.close()
-    one = _ctx_env({"SQ_BATCH": "1"}, lambda: sq.Context(d, name, device=0))
-    k = max(2, len(slabs))
-    dt1, n1, lz1, _ = timed(one, mode, sq.SQ_FP32, slabs, k, 1)
-    one.close()
-    out["kernel_factors"] = {
-        "precision": round((n1 / dt1) / value, 3),
-        "batching": round((nodes32 / dt32) / (n1 / dt1), 3),
-        "note": "nodes/s ratios on the same slabs: FP32 one-node kernel / FP64 one-node kernel, then FP32 batched / "
-                "FP32 one-node"}
+    if not args.no_factors:
+        one = _ctx_env({"SQ_BATCH": "1"}, lambda: sq.Context(d, name, device=0))
+        k = max(2, len(slabs))
+        dt1, n1, lz1, _ = timed(one, mode, sq.SQ_FP32, slabs, k, 1)
+        one.close()
+        out["kernel_factors"] = {
+            "precision": round((n1 / dt1) / value, 3),
+            "batching": round((nodes32 / dt32) / (n1 / dt1), 3),
+            "note": "nodes/s ratios on the same slabs: FP32 one-node kernel / FP64 one-node kernel, then FP32 "
+                    "batched / FP32 one-node"}
     if not args.no_scf:
         if args.scf_config and args.scf_config != args.config:
             ds, ns = decks.deck(args.scf_config, root)

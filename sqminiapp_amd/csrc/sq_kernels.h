@@ -69,6 +69,7 @@ struct KParams {
     const int2 *batch;
     const void *basis;           // (t0, t2, t2 z, t2 z^2) per image box point (element type = precision)
     int lds_run, lds_share;      // batched kernel: per-image atom run and type; per-atom shares of <v, V_nl v>
+    int plane_px;                // z-plane row pitch in LDS: the cube's nx plus halo and bank padding
 };
 
 hipError_t launch_lanczos(const KParams &P, int precision, int grid, size_t lds, hipStream_t st);

@@ -131,7 +131,7 @@ __global__ void __launch_bounds__(1024) lanczos_zbatch(KParams P)
     const int nx = P.nloc[0], ny = P.nloc[1], nz = P.nloc[2];
     const int ncol = nx * ny;
     const int nzs = nz + B - 1;                       // union planes of the storage
-    const int px = nx + 2 * O, pplane = px * (ny + 2 * O);
+    const int px = P.plane_px, pplane = px * (ny + 2 * O);
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int col = tid;
     const bool has = col < ncol;

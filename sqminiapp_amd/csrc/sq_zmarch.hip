@@ -91,7 +91,7 @@ lanczos_zmarch(KParams P)
     const int nx = P.nloc[0], ny = P.nloc[1], nz = P.nloc[2];
     const int ncol = nx * ny, nc = P.nc;
     // z-planes in LDS with a zero halo of O cells on every side in x and y
-    const int px = nx + 2 * O, pplane = px * (ny + 2 * O);
+    const int px = P.plane_px, pplane = px * (ny + 2 * O);
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     int col[CPT], ci[CPT], cj[CPT];
     bool has[CPT];
