@@ -187,6 +187,17 @@ int sq_set_early_exit(sq_ctx *ctx, double tol, double lambda_f, double beta, int
 /* Gauss nodes per node of the last sq_run (N_qp unless it stopped early). */
 int sq_get_rule_sizes(sq_ctx *ctx, int *n);
 
+/* The autotuner's rule (no counterpart in the reference), as sq_create_from_deck
+ * applies it per precision to its probe times: candidates 0/1 = the one-node
+ * kernel with 512/1024 threads, 2/3 = 2 nodes a workgroup with gather trip a/b,
+ * 4/5 = 4 nodes with trip a/b (trips 1/2 in FP64, 4/8 in FP32).  available[6]:
+ * which can launch; probe_ms[6]: their probe times; fixed_* the values of
+ * SQ_ZMARCH_WG, SQ_BATCH and SQ_BATCH_TRIP (NULL when unset), which narrow the
+ * set.  *candidate = the fastest remaining (lower index on a tie); candidate 0
+ * when the narrowing leaves nothing and it can launch; -1 when nothing can. */
+int sq_tune_select(int precision, const int *available, const float *probe_ms, const char *fixed_workgroup,
+                   const char *fixed_batch, const char *fixed_trip, int *candidate);
+
 /* Wait for all device work of the context's device (hipDeviceSynchronize). */
 int sq_synchronize(sq_ctx *ctx);
 

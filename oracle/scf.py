@@ -29,6 +29,20 @@ C_X = 0.75 * (3.0 / np.pi) ** (1.0 / 3.0)   # eps_x = -C_X rho^(1/3)
 PW92 = dict(A=0.031091, a1=0.21370, b1=7.5957, b2=3.5876, b3=1.6382, b4=0.49294)
 
 
+def pw92_eps_c(rs):
+    """(eps_c, d eps_c / d rs) of the unpolarised gas, PW92 eq. (10):
+    eps_c = -2A (1 + a1 rs) ln(1 + 1/Q), Q = 2A (b1 rs^1/2 + b2 rs + b3 rs^3/2 + b4 rs^2)."""
+    rs = np.asarray(rs, dtype=np.float64)
+    A, a1, b1, b2, b3, b4 = (PW92[k] for k in ("A", "a1", "b1", "b2", "b3", "b4"))
+    srs = np.sqrt(rs)
+    Q = 2.0 * A * (b1 * srs + b2 * rs + b3 * rs * srs + b4 * rs * rs)
+    dQ = 2.0 * A * (0.5 * b1 / srs + b2 + 1.5 * b3 * srs + 2.0 * b4 * rs)
+    L = np.log1p(1.0 / Q)
+    ec = -2.0 * A * (1.0 + a1 * rs) * L
+    dec = -2.0 * A * a1 * L + 2.0 * A * (1.0 + a1 * rs) * dQ / (Q * (Q + 1.0))
+    return ec, dec
+
+
 def xc_lda_pw(rho):
     """(eps_xc, v_xc) per point for the unpolarised LDA (Slater + PW92); zero where rho <= 0."""
     rho = np.asarray(rho, dtype=np.float64)
@@ -38,13 +52,7 @@ def xc_lda_pw(rho):
     r = rho[m]
     ex = -C_X * np.cbrt(r)
     rs = np.cbrt(3.0 / (4.0 * np.pi * r))
-    A, a1, b1, b2, b3, b4 = (PW92[k] for k in ("A", "a1", "b1", "b2", "b3", "b4"))
-    srs = np.sqrt(rs)
-    Q = 2.0 * A * (b1 * srs + b2 * rs + b3 * rs * srs + b4 * rs * rs)
-    dQ = 2.0 * A * (0.5 * b1 / srs + b2 + 1.5 * b3 * srs + 2.0 * b4 * rs)
-    L = np.log1p(1.0 / Q)
-    ec = -2.0 * A * (1.0 + a1 * rs) * L
-    dec = -2.0 * A * a1 * L + 2.0 * A * (1.0 + a1 * rs) * dQ / (Q * (Q + 1.0))
+    ec, dec = pw92_eps_c(rs)
     eps[m] = ex + ec
     v[m] = 4.0 / 3.0 * ex + ec - rs / 3.0 * dec
     return eps, v
@@ -133,10 +141,11 @@ class PeriodicPulay:
         return xn
 
 
-def scf(S, mode=1, tol=1e-6, maxit=60, beta=None, history=None, frequency=None, verbose=False, nthreads=8):
+def scf(S, mode=1, tol=1e-6, maxit=60, beta=None, history=None, frequency=None, verbose=False, nthreads=8,
+        callback=None):
     """Oracle SCF on an oracle System (oracle.System) with set_cell_period(True)
     and build_nonlocal(box=cell).  Returns a dict of the final fields, energies
-    and the residual history."""
+    and the residual history; callback(out), when given, sees each iteration's."""
     I = S.info()
     G = (I.npts[2] - 1, I.npts[1] - 1, I.npts[0] - 1)
     coeffs = [list(I.coeffs[d]) for d in range(3)]
@@ -168,6 +177,8 @@ def scf(S, mode=1, tol=1e-6, maxit=60, beta=None, history=None, frequency=None, 
                    residuals=res_hist)
         if verbose:
             print(f"oracle scf {it}: res {res:.3e} E {etot:.10f}", flush=True)
+        if callback is not None:
+            callback(out)
         if res < tol:
             out["converged"] = True
             return out

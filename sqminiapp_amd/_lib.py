@@ -24,7 +24,7 @@ STATUS = {0: "SQ_OK", 1: "SQ_ERR_IO", 2: "SQ_ERR_FORMAT", 3: "SQ_ERR_RANGE", 4: 
 # every symbol include/sq_amd.h declares
 EXPORTS = ["sq_last_error", "sq_default_options", "sq_create_from_deck", "sq_destroy", "sq_get_info", "sq_set_veff",
            "sq_run", "sq_get_nodes_and_weights", "sq_get_lanczos", "sq_nodes_and_weights", "sq_last_timing",
-           "sq_apply_h", "sq_apply_nonlocal", "sq_tridiag_nodes_weights", "sq_tridiag_nodes_weights_noalloc", "sq_density_fermi_energy", "sq_synchronize",
+           "sq_apply_h", "sq_apply_nonlocal", "sq_tridiag_nodes_weights", "sq_tridiag_nodes_weights_noalloc", "sq_tune_select", "sq_density_fermi_energy", "sq_synchronize",
            "sq_get_image", "sq_default_scf_options", "sq_scf", "sq_get_cell_field", "sq_set_veff_cell", "sq_xc",
            "sq_poisson", "sq_set_early_exit", "sq_get_rule_sizes", "sq_forces", "sq_mlp_create", "sq_mlp_destroy",
            "sq_mlp_get_params", "sq_mlp_set_params", "sq_mlp_descriptors", "sq_mlp_add_sample", "sq_mlp_add_sample_every",
@@ -138,6 +138,7 @@ def lib():
     L.sq_mlp_guess.argtypes = [vp, vp, C.c_int, dp]
     L.sq_tridiag_nodes_weights.argtypes = [vp, dp, dp, C.c_int, lg, dp, dp, dp, dp]
     L.sq_tridiag_nodes_weights_noalloc.argtypes = [vp, vp, vp, C.c_int, lg, vp, vp, vp, vp]
+    L.sq_tune_select.argtypes = [C.c_int, ip, C.POINTER(C.c_float), C.c_char_p, C.c_char_p, C.c_char_p, ip]
     L.sq_density_fermi_energy.argtypes = [vp, C.c_double, C.c_double, dp, dp, dp, dp]
     L.sq_synchronize.argtypes = [vp]
     L.sq_get_image.argtypes = [vp, C.c_int, dp, ip, ip, ip, ip, dp]
@@ -147,6 +148,16 @@ def lib():
 
 def _dp(a):
     return None if a is None else a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def tune_select(precision, available, probe_ms, workgroup=None, batch=None, trip=None):
+    """The autotuner's choice among its six candidates (sq_tune_select)."""
+    av = (C.c_int * 6)(*[int(bool(a)) for a in available])
+    ms = (C.c_float * 6)(*[float(x) for x in probe_ms])
+    enc = lambda v: None if v is None else str(v).encode()
+    out = C.c_int()
+    _check(lib().sq_tune_select(int(precision), av, ms, enc(workgroup), enc(batch), enc(trip), C.byref(out)))
+    return out.value
 
 
 def _check(rc):

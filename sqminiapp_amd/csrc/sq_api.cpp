@@ -719,6 +719,41 @@ static int probe_launch(sq_ctx *c, int prec, int B, int shape, int nqp, long npr
     return SQ_OK;
 }
 
+// The autotuner's rule.  Candidates: 0/1 the one-node kernel with 512/1024
+// threads, 2/3 B = 2 with gather trip a/b, 4/5 B = 4 with trip a/b.
+// available[k]: candidate k can launch on this deck.  The environment strings
+// (NULL when unset) narrow the set: SQ_ZMARCH_WG keeps the one-node kernel in
+// that shape, SQ_BATCH the batch size, SQ_BATCH_TRIP the trip of a batched
+// candidate.  Nothing left: candidate 0 when it can launch.
+static int tune_allowed(int prec, const int *available, const char *fixed_wg, const char *fixed_b,
+                        const char *fixed_t, bool allow[6])
+{
+    int n = 0;
+    for (int k = 0; k < 6; k++) {
+        const int B = k < 2 ? 1 : (k < 4 ? 2 : 4);
+        allow[k] = available[k] != 0;
+        if (fixed_wg) allow[k] = B == 1 && (atoi(fixed_wg) == 1024) == (k == 1) && available[k];
+        if (fixed_b && atoi(fixed_b) != B) allow[k] = false;
+        if (fixed_t && B > 1 && atoi(fixed_t) != batch_trip(prec, k & 1)) allow[k] = false;
+        n += allow[k];
+    }
+    if (n == 0 && available[0]) {
+        allow[0] = true;
+        n = 1;
+    }
+    return n;
+}
+
+// The fastest allowed candidate by its probe time (the lower index on a tie);
+// -1 when none is allowed.
+static int tune_pick(const bool allow[6], const float ms[6])
+{
+    int best = -1;
+    for (int k = 0; k < 6; k++)
+        if (allow[k] && (best < 0 || ms[k] < ms[best])) best = k;
+    return best;
+}
+
 // Grids, the Lanczos scratch sized for the largest configuration, and the
 // autotuner.  Per precision it times every configuration on one node set --
 // four planes of the rank box's middle (fewer if the box is thinner), the
@@ -784,33 +819,26 @@ int finish_device_setup(sq_ctx *c)
     if ((rc = dev_upload(&c->probe_ids, ids.data(), ids.size()))) return rc;
     const int nqp = std::min(S.nqp, 20);
     for (int prec = 0; prec < 2; prec++) {
-        // candidates: 0/1 one-node 512/1024 threads, 2/3 B=2 trips a/b, 4/5 B=4 trips a/b
-        bool allow[6];
+        int available[6];
         for (int k = 0; k < 6; k++) {
             const int B = k < 2 ? 1 : (k < 4 ? 2 : 4);
-            allow[k] = B == 1 ? grids[prec][k] > 0
-                              : (c->zbatch && zbatch_grid(c, prec, B, batch_trip(prec, k & 1)) > 0);
-            if (fixed_wg) allow[k] = B == 1 && (atoi(fixed_wg) == 1024) == (k == 1) && grids[prec][k] > 0;
-            if (fixed_b && atoi(fixed_b) != B) allow[k] = false;
-            if (fixed_t && B > 1 && atoi(fixed_t) != batch_trip(prec, k & 1)) allow[k] = false;
+            available[k] = B == 1 ? grids[prec][k] > 0
+                                  : (c->zbatch && zbatch_grid(c, prec, B, batch_trip(prec, k & 1)) > 0);
         }
-        int n_allowed = 0, best = -1;
-        for (int k = 0; k < 6; k++) n_allowed += allow[k];
-        if (n_allowed == 0) allow[0] = grids[prec][0] > 0, n_allowed = 1;
-        float best_ms = 0.f;
-        for (int k = 0; k < 6; k++) {
+        bool allow[6];
+        const int n_allowed = tune_allowed(prec, available, fixed_wg, fixed_b, fixed_t, allow);
+        float ms[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+        for (int k = 0; k < 6 && n_allowed > 1; k++) {
             if (!allow[k]) continue;
-            if (n_allowed == 1) { best = k; break; }
             const int B = k < 2 ? 1 : (k < 4 ? 2 : 4);
             if (B > 1 && (rc = upload_batches(c, bl[B]))) return rc;
-            float ms = 0.f;
             if ((rc = probe_launch(c, prec, B, B == 1 ? (k ? 1024 : 512) : batch_trip(prec, k & 1), nqp, (long)ids.size(),
-                                   &ms)))
+                                   &ms[k])))
                 return rc;
-            c->tune_cand[prec][k] = ms;
-            if (k < 2) c->tune_ms[prec][k] = ms;
-            if (best < 0 || ms < best_ms) { best = k; best_ms = ms; }
+            c->tune_cand[prec][k] = ms[k];
+            if (k < 2) c->tune_ms[prec][k] = ms[k];
         }
+        int best = tune_pick(allow, ms);
         if (best < 0) best = 0;
         if (best < 2) {
             c->zwg[prec] = best ? 1024 : 512;
@@ -848,6 +876,17 @@ int check_status(sq_ctx *c, long begin, long count, const char *what)
 extern "C" {
 
 const char *sq_last_error(void) { return g_err.c_str(); }
+
+int sq_tune_select(int precision, const int *available, const float *probe_ms, const char *fixed_workgroup,
+                   const char *fixed_batch, const char *fixed_trip, int *candidate)
+{
+    if (!available || !probe_ms || !candidate) return set_err(SQ_ERR_ARG, "null argument");
+    if (precision != SQ_FP64 && precision != SQ_FP32) return set_err(SQ_ERR_ARG, "precision must be SQ_FP64 or SQ_FP32");
+    bool allow[6];
+    tune_allowed(precision, available, fixed_workgroup, fixed_batch, fixed_trip, allow);
+    *candidate = tune_pick(allow, probe_ms);
+    return SQ_OK;
+}
 
 void sq_default_options(sq_options *opt)
 {

@@ -2,9 +2,15 @@
 density of oracle/sq_oracle.c) without the reference, which declares its SCF
 but does not build it:
 
-- LDA: Slater exchange in closed form, PW92 correlation against the
-  independent Perdew-Zunger 1981 fit of the same Ceperley-Alder data, and
-  v_xc = d(rho eps_xc)/d rho by central differences;
+- LDA: Slater exchange in closed form; PW92 correlation against its published
+  high- and low-density expansions (coefficients c0..c3, d0, d1 of Perdew &
+  Wang, Phys. Rev. B 45, 13244 (1992)) to 1e-6 Ha, against the independent
+  Vosko-Wilk-Nusair (VWN5) and Perdew-Zunger 1981 fits of the same
+  Ceperley-Alder data, and v_xc = d(rho eps_xc)/d rho by central differences;
+- Harris-Foulkes energy: along the oracle SCF it differs from the Kohn-Sham
+  free-energy functional of the output density (a different formula: kinetic
+  energy E_band - int V_in rho_out, Hartree and xc terms of rho_out) only at
+  second order in rho_out - rho_in, and the two meet at self-consistency;
 - Poisson: the FFT solve with the stencil's exact symbol recovers a
   manufactured potential whose right-hand side is built with the stencil in
   real space (np.roll), and the reverse;
@@ -35,6 +41,55 @@ def test_exchange_closed_form_and_pw92_against_pz81():
     pz = OS.pz81_eps_c(rs)
     assert np.all(np.abs(ec - pz) / np.abs(pz) < 0.03), (ec, pz)
     assert np.all(np.diff(ec) > 0)              # |eps_c| falls with rs
+
+
+def test_pw92_against_published_expansions():
+    """eps_c = c0 ln rs - c1 + c2 rs ln rs - c3 rs + O(rs^3/2) (rs -> 0) and
+    eps_c = -d0/rs + d1/rs^3/2 + O(rs^-2) (rs -> oo) with PW92's published
+    c0 = 0.031091 (= (1 - ln 2)/pi^2), c1 = 0.046644, c2 = 0.00664,
+    c3 = 0.01043, d0 = 0.4335, d1 = 1.4408."""
+    for rs in (1e-8, 1e-6, 1e-4, 1e-3):
+        ec, _ = OS.pw92_eps_c(rs)
+        assert abs(ec - (0.031091 * np.log(rs) - 0.046644 + 0.00664 * rs * np.log(rs) - 0.01043 * rs)) < 1e-6, rs
+    for rs in (1e4, 1e6, 1e8):
+        ec, _ = OS.pw92_eps_c(rs)
+        assert abs(ec - (-0.4335 / rs + 1.4408 * rs ** -1.5)) < 1e-3 * 0.4335 / rs, rs
+    P = OS.PW92
+    A, a1, b1, b2, b3, b4 = (P[k] for k in ("A", "a1", "b1", "b2", "b3", "b4"))
+    assert abs(A - (1.0 - np.log(2.0)) / np.pi ** 2) < 1e-6
+    # the parameter relations behind the limits: no sqrt(rs) term, c1, c2, c3, d0, d1
+    assert b2 == pytest.approx(2 * A * b1 * b1, rel=1e-4)
+    c1 = -2 * A * np.log(2 * A * b1)
+    assert abs(c1 - 0.046644) < 1e-6
+    assert A * a1 == pytest.approx(0.00664, rel=1e-3)
+    assert a1 * c1 + 2 * A * (4 * A * A * b1 * b1 - b3 / b1) == pytest.approx(0.01043, rel=1e-3)
+    assert a1 / b4 == pytest.approx(0.4335, rel=1e-4) and a1 * b3 / b4 ** 2 == pytest.approx(1.4408, rel=1e-4)
+    # d eps_c / d rs by central differences
+    rs = np.array([0.3, 1.0, 4.0, 12.0])
+    _, dec = OS.pw92_eps_c(rs)
+    fd = (OS.pw92_eps_c(rs * (1 + 1e-6))[0] - OS.pw92_eps_c(rs * (1 - 1e-6))[0]) / (2e-6 * rs)
+    assert np.allclose(dec, fd, rtol=1e-7)
+
+
+def _vwn5_eps_c(rs):
+    """Vosko-Wilk-Nusair (Can. J. Phys. 58, 1200 (1980)) paramagnetic fit 'V',
+    in hartree: A = 0.0310907, x0 = -0.10498, b = 3.72744, c = 12.9352."""
+    A, x0, b, c = 0.0310907, -0.10498, 3.72744, 12.9352
+    x = np.sqrt(rs)
+    X = lambda t: t * t + b * t + c
+    Q = np.sqrt(4 * c - b * b)
+    at = np.arctan(Q / (2 * x + b))
+    return A * (np.log(x * x / X(x)) + 2 * b / Q * at
+                - b * x0 / X(x0) * (np.log((x - x0) ** 2 / X(x)) + 2 * (b + 2 * x0) / Q * at))
+
+
+def test_pw92_against_vwn5():
+    """Two fits of the same Ceperley-Alder data agree within 0.7 % over the
+    densities of a metal (rs 0.5-20): a wrong PW92 parameter or term would not."""
+    rs = np.array([0.5, 0.8, 1.0, 1.5, 2.0, 3.0, 4.0, 5.0, 8.0, 10.0, 20.0])
+    ec, _ = OS.pw92_eps_c(rs)
+    vw = _vwn5_eps_c(rs)
+    assert np.all(np.abs(ec - vw) / np.abs(vw) < 7e-3), (ec - vw) / np.abs(vw)
 
 
 def test_vxc_is_the_density_derivative():
@@ -184,6 +239,44 @@ def test_oracle_scf_converges_on_a_small_cube_deck(tmp_path):
     assert out["rho_out"].sum() * I.dV == pytest.approx(8.0, rel=1e-10)
     # self-consistency: the last input density reproduces itself
     assert np.linalg.norm(out["rho_out"] - out["rho_in"]) / np.linalg.norm(out["rho_out"]) < 1e-6
+
+
+def test_harris_foulkes_against_the_kohn_sham_functional(tmp_path):
+    """E_HF(rho_in) = E_band + E_xc[rho_in] - int V_xc rho_in + 1/2 int phi_in (b - rho_in) - E_self - TS
+    against F_KS(rho_out) = (E_band - int V_in rho_out) + E_xc[rho_out] + 1/2 int (rho_out + b) phi_out
+    - E_self - TS, phi_out by FFT from rho_out + b.  Their difference is second order in
+    rho_out - rho_in: a wrong sign, a dropped or doubled term leaves a first-order or a constant gap."""
+    from oracle import oracle as O
+    d, n = write_variant(str(tmp_path), "T", rcut=1.3)
+    S = O.System(d, n)
+    S.set_phantom(False)
+    S.set_cell_period(True)
+    S.build_nonlocal(S.cell())
+    I = S.info()
+    coeffs, order = [list(I.coeffs[k]) for k in range(3)], I.order
+    b = S.pseudocharge()[0].reshape(I.npts[2] - 1, I.npts[1] - 1, I.npts[0] - 1)
+    rows = []
+
+    def gap(o):
+        ri, ro = o["rho_in"], o["rho_out"]
+        eps_o, _ = OS.xc_lda_pw(ro)
+        phi_o = OS.poisson_fft(coeffs, order, ro + b)
+        _, vxc_i = OS.xc_lda_pw(ri)
+        fks = (o["e_band"] - np.sum((o["phi"] + vxc_i) * ro) * I.dV + np.sum(eps_o * ro) * I.dV
+               + 0.5 * np.sum((ro + b) * phi_o) * I.dV - o["e_self"] + o["minus_ts"])
+        rows.append((np.linalg.norm(ro - ri) * np.sqrt(I.dV), fks - o["e_total"], o["e_total"]))
+
+    out = OS.scf(S, mode=1, tol=1e-7, maxit=40, beta=0.3, history=7, frequency=2, callback=gap)
+    assert out["converged"]
+    dn = np.array([r[0] for r in rows])
+    de = np.abs([r[1] for r in rows])
+    print("||rho_out - rho_in||, |F_KS - E_HF|:", [(f"{a:.1e}", f"{c:.1e}") for a, c in zip(dn, de)])
+    assert de[-1] < 1e-10 * abs(rows[-1][2])
+    # second order: |F_KS - E_HF| / ||delta||^2 bounded as ||delta|| falls by decades
+    k = de / dn ** 2
+    early = k[dn > 1e-2].max()
+    assert np.all(k[dn < 1e-3] < 10 * early), k
+    assert de[0] > 1e-6 * abs(rows[0][2])      # the comparison is not vacuous
 
 
 def test_kerker_symbol_on_plane_waves(mg1_oracle):
