@@ -3,8 +3,9 @@
 Workload (default): the 128-atom hcp Mg cell with a 4-site vacancy cluster
 (49x42x79 = 162,582 grid nodes, 29^3 cubes, 80 Lanczos steps, KB projectors).
 One step = one SQ pass (N_qp+1 H applications + QL per node) over one slab
-of whole z-planes: the grid is streamed as 8 slabs of ~20k nodes, step i
-taking slab i mod 8, inputs resident in HBM.  The headline value is the
+of 12 whole z-planes (a multiple of the batched kernel's 2 and 4 nodes a
+column; 24,696 nodes, the last slab 7 planes): the grid is streamed slab by
+slab, step i taking slab i mod 7, inputs resident in HBM.  The headline value is the
 plain FP64 form (the one-node column-marching kernel); the chosen FP32 form
 (batched kernel, MFMA projector gather) is measured on the same slabs beside
 it, with the kernel factors of precision and batching separated.
@@ -38,7 +39,7 @@ sys.path.insert(0, REPO)
 
 METRIC = "SQ grid-nodes/sec + SCF wall-time at <=1e-5 Ha/atom, chosen vs plain FP64, 1 GPU"
 HBM_PEAK_GBS = 8000.0
-N_SLABS = 8
+PLANES_PER_SLAB = 12   # a multiple of the batch sizes 2 and 4
 KERKER_K0 = 1.0       # Kerker wavenumber, 1/bohr
 KERKER_MIX = 0.7      # Pulay mixing parameter with the Kerker preconditioner
 SCF_TOL = 1e-5        # the common stop of every timed form (the FP32 floor is about 5e-6)
@@ -104,12 +105,10 @@ def _cpu_run(deck_dir, name, fast, seconds):
 
 def slab_ranges(I):
     lx, ly, lz = I.rank_box[3], I.rank_box[4], I.rank_box[5]
-    pps = -(-lz // N_SLABS)
     out = []
-    for s in range(N_SLABS):
-        k0, k1 = s * pps, min(lz, (s + 1) * pps)
-        if k0 < k1:
-            out.append((k0 * lx * ly, (k1 - k0) * lx * ly))
+    for k0 in range(0, lz, PLANES_PER_SLAB):
+        k1 = min(lz, k0 + PLANES_PER_SLAB)
+        out.append((k0 * lx * ly, (k1 - k0) * lx * ly))
     return out
 
 
@@ -213,8 +212,9 @@ def scf_section(d, name, mode, max_iter, budget_s, verbose=False):
         "plain_fp64_deck_mixing": {
             "form": "FP64 Lanczos, the deck's Periodic Pulay (mixing 0.3, history 7)",
             "converged": False,
-            "note": "not converged in 80 iterations (over 700 s; the residual sloshes between 0.03 and 0.1); "
-                    "recorded in profiles/scf_c128_summary.md, not repeated"},
+            "note": "not converged: 30 iterations in 272 s, the residual sloshing between 0.035 and 0.59 "
+                    "(python -m sqminiapp_amd.scf_run --config C128 --precision fp64 --mixing deck --max-iter 30); "
+                    "recorded in profiles/scf_c128_summary.md, not repeated here"},
         "chosen_form": {
             "form": f"FP32 Lanczos, batched kernel (MFMA projector gather), {kmix}, SCF step replayed from one captured "
                     f"graph, " + ("start from the learned initial density (BF16 network inference)" if learned
@@ -293,7 +293,7 @@ def main():
         "warmup": args.warmup, "ms_per_step": round(dt64 * 1e3 / args.steps, 3), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": f"{args.config}: {I.nAtoms}-atom hcp Mg cell, grid {I.npts[0]}x{I.npts[1]}x"
-                               f"{I.npts[2]} = {n_nodes} nodes streamed as {len(slabs)} slabs of whole z-planes "
+                               f"{I.npts[2]} = {n_nodes} nodes streamed as {len(slabs)} slabs of {PLANES_PER_SLAB} whole z-planes "
                                f"(a step = one slab's SQ pass), {I.nloc[0]}^3 cubes, {I.nqp} Lanczos steps, "
                                f"{args.mode} nonlocal, V_eff 0.333",
                    "nodes": n_nodes, "slab_nodes": [c for _, c in slabs], "nqp": I.nqp, "cube": I.nloc[0],
