@@ -445,8 +445,8 @@ int create_device(sq_ctx *c, int device)
     c->zmarch = S.order == 6 && ncol <= 1024 && c->img_cap <= 256 && getenv("SQ_FORCE_GENERIC") == nullptr;
     // Row pitch of the LDS planes: the stencil needs a halo of 6 on each side
     // (nx + 12).  Padding the pitch to nx + 32 keeps lanes that cross a row end
-    // on consecutive bank slots; on the 128-atom cell it gains 1-3 % (FP64
-    // one-node 16.49k -> 16.95k nodes/s, FP32 batched 29.2k -> 30.0k), the
+    // on consecutive bank slots; on the 128-atom cell it gains 3 % (FP64
+    // one-node 16.48k -> 16.97k nodes/s, FP32 batched 34.4k -> 35.3k), the
     // results identical bit for bit.  SQ_PLANE_PAD (12..64) overrides it.
     {
         int pad = 32;

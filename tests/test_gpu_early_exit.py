@@ -62,4 +62,6 @@ def test_scf_with_early_exit_converges():
     de = abs(r["e_total"] - r0["e_total"]) / 4
     print("Mg1 SCF with early exit:", r["iterations"], "iterations, mean rule", r["mean_rule_size"], "dE/atom", de)
     assert r["converged"] and r["mean_rule_size"] <= 10 and r0["mean_rule_size"] == 10
-    assert de < 1e-3
+    # at 10 Gauss nodes the exit test rarely fires (measured: no node stops, dE = 0); the chosen SCF
+    # form runs without the exit, and any stop here must keep the metric's 1e-5 Ha/atom
+    assert de < 1e-5

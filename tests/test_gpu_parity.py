@@ -108,6 +108,33 @@ def test_mg1_nodes_weights_density_kb(ctx, golden, oracle_mod):
     assert lf == pytest.approx(lf2, abs=1e-12)
 
 
+@pytest.mark.parametrize("wg", ["512", "1024"])
+@pytest.mark.parametrize("prec", [sq.SQ_FP64, sq.SQ_FP32])
+def test_mg1_each_one_node_shape_against_golden(monkeypatch, golden, wg, prec):
+    """The one-node kernel pinned to each workgroup shape (SQ_BATCH=1,
+    SQ_ZMARCH_WG), whichever the autotuner would pick, on all 3366 nodes."""
+    monkeypatch.setenv("SQ_BATCH", "1")
+    monkeypatch.setenv("SQ_ZMARCH_WG", wg)
+    with sq.Context(MG1, "Mg1", phantom_atom=True) as c:
+        monkeypatch.delenv("SQ_BATCH")
+        monkeypatch.delenv("SQ_ZMARCH_WG")
+        assert c.info.workgroup[prec] == int(wg) and c.info.batch[prec] == 1
+        c.run(sq.SQ_NL_KB, prec)
+        lam, w, lmin, lmax = c.results()
+        lf, rho, eb, mts = c.density_fermi_energy()
+    ref = np.sort(golden["kb_lam"], 1)
+    if prec == sq.SQ_FP64:
+        assert rel_err(np.sort(lam, 1), ref) < 1e-12
+        assert np.max(np.abs(lmin - golden["kb_lmin"])) < 1e-11 and np.max(np.abs(lmax - golden["kb_lmax"])) < 1e-11
+        assert lf == pytest.approx(float(golden["kb_lf"]), abs=1e-11)
+        assert rel_err(rho, golden["kb_rho"]) < 1e-10
+    else:
+        assert rel_err(np.sort(lam, 1), ref) < 2e-6
+        assert abs(lf - float(golden["kb_lf"])) < 1e-6
+        assert abs(eb - float(golden["kb_eband"])) / 5 < 1e-5
+        assert rel_err(rho, golden["kb_rho"]) < 5e-5
+
+
 def test_mg1_collapsed_nodes_match_oracle(ctx, golden):
     ctx.run(sq.SQ_NL_REF_COLLAPSED, sq.SQ_FP64)
     lam, w, lmin, lmax = ctx.results()
