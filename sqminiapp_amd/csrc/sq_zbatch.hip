@@ -15,16 +15,17 @@
 // outside node b's cube hold exact zeros, which is the zero-Dirichlet
 // truncation of its cube in z.
 //
-// The projector gather is a batched small GEMM on the matrix cores: for one
-// image and one (x,y) column of its box, S[m][n] = sum_z basis_m(z) v_n(z)
-// with the four column basis functions t0, t2, t2 z, t2 z^2 (the s+d channels
-// are t0 and t2 times a quadratic in z with column constants) and n the
-// batch's nodes.  FP32: v_mfma_f32_4x4x1_16b (16 columns per instruction,
-// one plane); FP64: v_mfma_f64_4x4x4_4b (4 columns, 4 planes).  The six
-// channel sums of a node follow from S and the column constants, then a
-// butterfly over the columns.
+// The projector gather is a batched small GEMM: for one image and one (x,y)
+// column of its box, S[m][n] = sum_z basis_m(z) v_n(z) with the four column
+// basis functions t0, t2, t2 z, t2 z^2 (the s+d channels are t0 and t2 times
+// a quadratic in z with column constants) and n the batch's nodes.  FP32 runs
+// it on the matrix cores, v_mfma_f32_4x4x1_16b (16 columns per instruction,
+// one plane).  FP64 runs it on the vector ALU, one column per lane: the FP64
+// matrix instruction (v_mfma_f64_4x4x4_4b) runs at the FP64 vector rate and
+// was measured 10-35 % slower there.  The six channel sums of a node follow
+// from S and the column constants, then a butterfly over the columns.
 //
-// Per H application: gather (MFMA) -> alpha per node -> local march (stencil
+// Per H application: gather -> alpha per node -> local march (stencil
 // from a 13-deep register ring per node and an LDS plane of the batch) ->
 // scatter of chi alpha per column and node + three-term update.
 #include <type_traits>
@@ -96,7 +97,7 @@ __device__ __forceinline__ void block_csum_vec(CSum c[B], double v[B], double *r
 }
 
 // One workgroup of 1024 threads per batch, one (x,y) column per thread.
-// UG: z-planes per gather trip (FP32; the FP64 instruction takes 4 planes),
+// UG: z-planes per gather trip,
 // UC: planes per scatter/update chunk.
 template <typename T, int B, int MW, int UG, int UC>
 __global__ void __launch_bounds__(1024) lanczos_zbatch(KParams P)
@@ -285,9 +286,7 @@ __global__ void __launch_bounds__(1024) lanczos_zbatch(KParams P)
                 const int zs2 = dm.x * dm.y;
                 // the basis tables hold z relative to the image centre, so a node's
                 // values depend neither on where the batch's union starts nor on how
-                // a rank box clips the image box; FP64 groups planes by 4 from the
-                // grid plane nearest the centre for the same reason
-                const int pcz = (int)floor((cen.z + P.range[2]) / P.hgrid[2] + 0.5);
+                // a rank box clips the image box
                 const int j = lane & 3;                 // basis index (A) and node index (B, D columns)
                 T acc[NLM];
 #pragma unroll
@@ -332,52 +331,78 @@ __global__ void __launch_bounds__(1024) lanczos_zbatch(KParams P)
                         }
                     }
                 } else {
-                    // 4 columns per instruction (block = (lane / 4) % 4), 4 planes (k = lane / 16);
-                    // the groups of 4 planes are aligned to the centre plane (zal), so a
-                    // node's sums are grouped alike whatever batch or rank box it runs in
-                    const int blk = (lane >> 2) & 3, k = lane >> 4;
-                    const int zal = pcz + ((a0[2] - pcz) & ~3);
-                    const int skip = a0[2] - zal, nzal = a0[2] + n[2] - zal;
-                    for (int q0 = 0; q0 < nxy; q0 += 4) {
-                        const int q = q0 + blk;
+                    // FP64: one column of the box per lane, UZ planes a trip, on the vector
+                    // ALU (the FP64 matrix instruction runs at the FP64 vector rate and
+                    // needs a lane per node and basis function).  Each node's four basis
+                    // sums run over z in ascending order from the box's first plane in the
+                    // union; planes outside a node's own cube hold exact zeros, so a
+                    // node's sums do not depend on the batch it runs in.
+                    constexpr int UZ = UG;
+                    T accb[B][NLM];
+#pragma unroll
+                    for (int b = 0; b < B; b++)
+#pragma unroll
+                        for (int ch = 0; ch < NLM; ch++) accb[b][ch] = T(0);
+                    const V4 *bas4 = reinterpret_cast<const V4 *>(bas + 2 * toff[li]);
+                    for (int q0 = 0; q0 < nxy; q0 += 64) {
+                        const int q = q0 + lane;
                         const bool ok = q < nxy;
                         const int yi = ok ? fdiv(q, n[0], inv0) : 0;
                         const int xi = ok ? q - yi * n[0] : 0;
                         const int gx = a0[0] + xi, gy = a0[1] + yi;
                         const int c = (gy - cube_lo[1]) * nx + (gx - cube_lo[0]);
-                        const T *vp = Bv + ((int64_t)(zal - cube_lo[2] + k) * ncol + c) * B + (j < B ? j : 0);
-                        const T *rp = bas + 2 * toff[li] +
-                                      ((int64_t)((zal + k - lo.z) * dm.y + (gy - lo.y)) * dm.x + (gx - lo.x)) * 4 + j;
-                        double D = 0.0;
-#pragma unroll 1
-                        for (int zi = 0; zi < nzal; zi += 4 * UG) {
-                            T tv[UG], vv[UG];
+                        const VB *vp = reinterpret_cast<const VB *>(Bv) + (size_t)(a0[2] - cube_lo[2]) * ncol + c;
+                        const V4 *rp = bas4 + ((int64_t)(a0[2] - lo.z) * dm.y + (gy - lo.y)) * dm.x + (gx - lo.x);
+                        T sm[B][4];
 #pragma unroll
-                            for (int u = 0; u < UG; u++) {
-                                const int zq = zi + 4 * u + k;
-                                const bool in = ok && zq >= skip && zq < nzal;
-                                tv[u] = in ? rp[(int64_t)(zi + 4 * u) * zs2 * 4] : T(0);
-                                vv[u] = (in && j < B) ? vp[(int64_t)(zi + 4 * u) * ncol * B] : T(0);
+                        for (int b = 0; b < B; b++)
+#pragma unroll
+                            for (int m = 0; m < 4; m++) sm[b][m] = T(0);
+#pragma unroll 1
+                        for (int zi = 0; zi < n[2]; zi += UZ) {
+                            V4 tb[UZ];
+                            VB vv[UZ];
+#pragma unroll
+                            for (int u = 0; u < UZ; u++) {
+                                const bool in = ok && zi + u < n[2];
+                                tb[u] = in ? rp[(int64_t)(zi + u) * zs2] : V4{};
+                                vv[u] = in ? vp[(size_t)(zi + u) * ncol] : VB{};
                             }
 #pragma unroll
-                            for (int u = 0; u < UG; u++) D = __builtin_amdgcn_mfma_f64_4x4x4f64(tv[u], vv[u], D, 0, 0, 0);
+                            for (int u = 0; u < UZ; u++)
+#pragma unroll
+                                for (int b = 0; b < B; b++) {
+                                    const T vb = vv[u][b];
+#pragma unroll
+                                    for (int m = 0; m < 4; m++) sm[b][m] = fma(tb[u][m], vb, sm[b][m]);
+                                }
                         }
-                        // D = S[basis k][node j] of column q (the D row index is lane / 16)
-                        if (ok && j < B) {
+                        if (ok) {
                             const T x = (T)(P.hgrid[0] * gx - P.range[0] - cen.x);
                             const T y = (T)(P.hgrid[1] * gy - P.range[1] - cen.y);
                             const T C = T(1.092548430592079), C20 = T(0.315391565252520), C22 = T(0.546274215296040);
-                            if (k == 0) acc[0] += D;
-                            else if (k == 1) {
-                                acc[1] = fma(C * (x * y), D, acc[1]);
-                                acc[3] = fma(C20 * (-(x * x) - y * y), D, acc[3]);
-                                acc[5] = fma(C22 * (x * x - y * y), D, acc[5]);
-                            } else if (k == 2) {
-                                acc[2] = fma(C * y, D, acc[2]);
-                                acc[4] = fma(C * x, D, acc[4]);
-                            } else acc[3] = fma(T(2) * C20, D, acc[3]);
+#pragma unroll
+                            for (int b = 0; b < B; b++) {
+                                accb[b][0] += sm[b][0];
+                                accb[b][1] = fma(C * (x * y), sm[b][1], accb[b][1]);
+                                accb[b][2] = fma(C * y, sm[b][2], accb[b][2]);
+                                accb[b][3] = fma(C20, fma(-(x * x) - y * y, sm[b][1], T(2) * sm[b][3]), accb[b][3]);
+                                accb[b][4] = fma(C * x, sm[b][2], accb[b][4]);
+                                accb[b][5] = fma(C22 * (x * x - y * y), sm[b][1], accb[b][5]);
+                            }
                         }
                     }
+                    // butterfly over all lanes, node by node; lane 0 writes
+#pragma unroll
+                    for (int b = 0; b < B; b++)
+#pragma unroll
+                        for (int ch = 0; ch < NLM; ch++) {
+                            double sv = (double)accb[b][ch];
+#pragma unroll
+                            for (int off = 1; off < 64; off <<= 1) sv += __shfl_xor(sv, off, 64);
+                            if (lane == 0 && ch < nlm) A[(li * B + b) * NLM + ch] = sv * (double)(T)s_sv[b];
+                        }
+                    continue;
                 }
                 // butterfly over the lanes of one node (lane bits 2..5); lanes 0..B-1 end with node lane
 #pragma unroll
@@ -771,7 +796,7 @@ static int occ_b(const KParams &P, size_t lds)
 }
 
 // The tuned shapes: FP64 batches of 2 nodes, FP32 batches of 2 or 4; the
-// gather trip (UG) of 4 or 8 planes (FP32) or 1 or 2 four-plane steps (FP64).
+// gather trip (UG) of 4 or 8 planes (FP32) or 1 or 2 planes (FP64).
 bool zbatch_supported(int precision, int B, int ug)
 {
     if (precision == 0) return B == 2 && (ug == 1 || ug == 2);
