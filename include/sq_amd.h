@@ -190,7 +190,8 @@ int sq_get_rule_sizes(sq_ctx *ctx, int *n);
 /* The autotuner's rule (no counterpart in the reference), as sq_create_from_deck
  * applies it per precision to its probe times: candidates 0/1 = the one-node
  * kernel with 512/1024 threads, 2/3 = 2 nodes a workgroup with gather trip a/b,
- * 4/5 = 4 nodes with trip a/b (trips 1/2 in FP64, 4/8 in FP32).  available[6]:
+ * 4/5 = 4 nodes with trip a/b (planes a trip: 1/2 in FP64 on the vector ALU;
+ * 4 on the matrix cores / 2 on the vector ALU in FP32).  available[6]:
  * which can launch; probe_ms[6]: their probe times; fixed_* the values of
  * SQ_ZMARCH_WG, SQ_BATCH and SQ_BATCH_TRIP (NULL when unset), which narrow the
  * set.  *candidate = the fastest remaining (lower index on a tie); candidate 0

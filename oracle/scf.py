@@ -39,7 +39,8 @@ def pw92_eps_c(rs):
     dQ = 2.0 * A * (0.5 * b1 / srs + b2 + 1.5 * b3 * srs + 2.0 * b4 * rs)
     L = np.log1p(1.0 / Q)
     ec = -2.0 * A * (1.0 + a1 * rs) * L
-    dec = -2.0 * A * a1 * L + 2.0 * A * (1.0 + a1 * rs) * dQ / (Q * (Q + 1.0))
+    with np.errstate(over="ignore"):   # Q (Q + 1) overflows only where the term is 0 anyway
+        dec = -2.0 * A * a1 * L + 2.0 * A * (1.0 + a1 * rs) * dQ / (Q * (Q + 1.0))
     return ec, dec
 
 

@@ -649,8 +649,10 @@ static int zmarch_grid(sq_ctx *c, int prec, int wg)
     return occ <= 0 ? 0 : std::max(8, (c->n_cu * occ) & ~7);
 }
 
-// The gather trips the batched kernel is built with, per precision: a, b.
-static int batch_trip(int prec, int which) { return prec == 0 ? 1 + which : 4 + 4 * which; }
+// The gather trips the batched kernel is built with, per precision: a, b
+// (FP64: 1 or 2 planes on the vector ALU; FP32: 4 planes on the matrix cores
+// or 2 on the vector ALU).
+static int batch_trip(int prec, int which) { return prec == 0 ? 1 + which : 4 - 2 * which; }
 
 static int zbatch_grid(sq_ctx *c, int prec, int B, int ug)
 {

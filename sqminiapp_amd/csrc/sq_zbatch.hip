@@ -18,12 +18,15 @@
 // The projector gather is a batched small GEMM: for one image and one (x,y)
 // column of its box, S[m][n] = sum_z basis_m(z) v_n(z) with the four column
 // basis functions t0, t2, t2 z, t2 z^2 (the s+d channels are t0 and t2 times
-// a quadratic in z with column constants) and n the batch's nodes.  FP32 runs
-// it on the matrix cores, v_mfma_f32_4x4x1_16b (16 columns per instruction,
-// one plane).  FP64 runs it on the vector ALU, one column per lane: the FP64
-// matrix instruction (v_mfma_f64_4x4x4_4b) runs at the FP64 vector rate and
-// was measured 10-35 % slower there.  The six channel sums of a node follow
-// from S and the column constants, then a butterfly over the columns.
+// a quadratic in z with column constants) and n the batch's nodes.  Two
+// engines, chosen per deck by the autotuner through the gather trip: FP32
+// trips of 4 planes run it on the matrix cores, v_mfma_f32_4x4x1_16b (16
+// columns per instruction, one plane a step); trips of 1 or 2 planes (FP64,
+// and FP32 trip 2) on the vector ALU, one column per lane.  The FP64 matrix
+// instruction (v_mfma_f64_4x4x4_4b) runs at the FP64 vector rate; it was
+// measured 10-35 % slower than the vector form and is not built.  The six
+// channel sums of a node follow from S and the column constants, then a
+// butterfly over the columns.
 //
 // Per H application: gather -> alpha per node -> local march (stencil
 // from a 13-deep register ring per node and an LDS plane of the batch) ->
@@ -291,8 +294,8 @@ __global__ void __launch_bounds__(1024) lanczos_zbatch(KParams P)
                 T acc[NLM];
 #pragma unroll
                 for (int ch = 0; ch < NLM; ch++) acc[ch] = T(0);
-                if constexpr (sizeof(T) == 4) {
-                    // 16 columns per instruction (block = lane / 4), one plane per step
+                if constexpr (sizeof(T) == 4 && UG >= 4) {
+                    // matrix cores: 16 columns per instruction (block = lane / 4), one plane per step
                     const int g = lane >> 2;
                     for (int q0 = 0; q0 < nxy; q0 += 16) {
                         const int q = q0 + g;
@@ -331,12 +334,10 @@ __global__ void __launch_bounds__(1024) lanczos_zbatch(KParams P)
                         }
                     }
                 } else {
-                    // FP64: one column of the box per lane, UZ planes a trip, on the vector
-                    // ALU (the FP64 matrix instruction runs at the FP64 vector rate and
-                    // needs a lane per node and basis function).  Each node's four basis
-                    // sums run over z in ascending order from the box's first plane in the
-                    // union; planes outside a node's own cube hold exact zeros, so a
-                    // node's sums do not depend on the batch it runs in.
+                    // vector ALU: one column of the box per lane, UG planes a trip.  Each
+                    // node's four basis sums run over z in ascending order from the box's
+                    // first plane in the union; planes outside a node's own cube hold
+                    // exact zeros, so a node's sums do not depend on the batch it runs in.
                     constexpr int UZ = UG;
                     T accb[B][NLM];
 #pragma unroll
@@ -796,11 +797,12 @@ static int occ_b(const KParams &P, size_t lds)
 }
 
 // The tuned shapes: FP64 batches of 2 nodes, FP32 batches of 2 or 4; the
-// gather trip (UG) of 4 or 8 planes (FP32) or 1 or 2 planes (FP64).
+// gather trip (UG) of 1 or 2 planes on the vector ALU (FP64), 4 planes on the
+// matrix cores or 2 on the vector ALU (FP32).
 bool zbatch_supported(int precision, int B, int ug)
 {
     if (precision == 0) return B == 2 && (ug == 1 || ug == 2);
-    return (B == 2 || B == 4) && (ug == 4 || ug == 8);
+    return (B == 2 || B == 4) && (ug == 4 || ug == 2);
 }
 
 hipError_t launch_zbatch(const KParams &P, int precision, int B, int ug, int grid, size_t lds, hipStream_t st)
@@ -811,9 +813,9 @@ hipError_t launch_zbatch(const KParams &P, int precision, int B, int ug, int gri
         if (B == 2 && ug == 2) return launch_b<double, 2, 2>(P, grid, lds, st);
     } else {
         if (B == 2 && ug == 4) return launch_b<float, 2, 4>(P, grid, lds, st);
-        if (B == 2 && ug == 8) return launch_b<float, 2, 8>(P, grid, lds, st);
+        if (B == 2 && ug == 2) return launch_b<float, 2, 2>(P, grid, lds, st);
         if (B == 4 && ug == 4) return launch_b<float, 4, 4>(P, grid, lds, st);
-        if (B == 4 && ug == 8) return launch_b<float, 4, 8>(P, grid, lds, st);
+        if (B == 4 && ug == 2) return launch_b<float, 4, 2>(P, grid, lds, st);
     }
     return hipErrorInvalidValue;
 }
@@ -826,9 +828,9 @@ int zbatch_occupancy(const KParams &P, int precision, int B, int ug, size_t lds)
         if (B == 2 && ug == 2) return occ_b<double, 2, 2>(P, lds);
     } else {
         if (B == 2 && ug == 4) return occ_b<float, 2, 4>(P, lds);
-        if (B == 2 && ug == 8) return occ_b<float, 2, 8>(P, lds);
+        if (B == 2 && ug == 2) return occ_b<float, 2, 2>(P, lds);
         if (B == 4 && ug == 4) return occ_b<float, 4, 4>(P, lds);
-        if (B == 4 && ug == 8) return occ_b<float, 4, 8>(P, lds);
+        if (B == 4 && ug == 2) return occ_b<float, 4, 2>(P, lds);
     }
     return 0;
 }

@@ -1,7 +1,8 @@
 """The autotuner's selection rule (sq_tune_select, the rule sq_create_from_deck
 applies to its probe times), on the CPU: candidates 0/1 = one-node kernel with
 512/1024 threads, 2/3 = 2 nodes a workgroup with gather trip a/b, 4/5 = 4 nodes
-with trip a/b (FP64 trips 1/2, FP32 trips 4/8); the environment overrides
+with trip a/b (FP64 trips 1/2 planes, FP32 trips 4 planes on the matrix cores / 2
+on the vector ALU); the environment overrides
 SQ_ZMARCH_WG, SQ_BATCH and SQ_BATCH_TRIP narrow the set."""
 import pytest
 
@@ -41,7 +42,7 @@ def test_batch_and_trip_overrides():
     assert sq.tune_select(sq.SQ_FP32, ALL, ms, batch=1) == 1
     assert sq.tune_select(sq.SQ_FP32, ALL, ms, batch=2) == 3
     assert sq.tune_select(sq.SQ_FP32, ALL, ms, batch=4) == 4
-    assert sq.tune_select(sq.SQ_FP32, ALL, ms, batch=4, trip=8) == 5
+    assert sq.tune_select(sq.SQ_FP32, ALL, ms, batch=4, trip=2) == 5
     assert sq.tune_select(sq.SQ_FP32, ALL, ms, batch=2, trip=4) == 2
     assert sq.tune_select(sq.SQ_FP64, FP64_AV, [5.0, 4.0, 3.0, 6.0, 0.0, 0.0], batch=2, trip=2) == 3
     # a trip that the precision is not built with leaves the one-node kernels

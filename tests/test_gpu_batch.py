@@ -21,7 +21,8 @@ from sqminiapp_amd import decks
 
 pytestmark = pytest.mark.gpu
 
-CONFIGS = [(sq.SQ_FP64, 2, 1), (sq.SQ_FP64, 2, 2), (sq.SQ_FP32, 2, 4), (sq.SQ_FP32, 4, 4), (sq.SQ_FP32, 4, 8)]
+CONFIGS = [(sq.SQ_FP64, 2, 1), (sq.SQ_FP64, 2, 2), (sq.SQ_FP32, 2, 4), (sq.SQ_FP32, 2, 2), (sq.SQ_FP32, 4, 4),
+           (sq.SQ_FP32, 4, 2)]
 
 
 def _ctx(monkeypatch, B, trip=None, deck=(MG1, "Mg1"), **kw):
