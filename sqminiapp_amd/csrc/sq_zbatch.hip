@@ -428,6 +428,7 @@ __global__ void __launch_bounds__(1024) lanczos_zbatch(KParams P)
                 const int nlm = tdim[li].w & 255;
                 const double *gam = P.gamma_ch + run.z * kMaxChannels;
                 double share = 0.0;
+                double a6[NLM];
                 if (P.mode == 0) {   // one coefficient per atom
                     double al = 0.0, sa = 0.0;
                     for (int q = s; q <= t; q++)
@@ -436,8 +437,7 @@ __global__ void __launch_bounds__(1024) lanczos_zbatch(KParams P)
                             al += gam[ch] * x;
                             sa += x;
                         }
-                    for (int q = s; q <= t; q++)
-                        for (int ch = 0; ch < NLM; ch++) alpha[(q * B + b) * NLM + ch] = al;
+                    for (int ch = 0; ch < NLM; ch++) a6[ch] = al;
                     share = al * sa;
                 } else {             // per (l,m) channel
                     for (int ch = 0; ch < NLM; ch++) {
@@ -447,8 +447,22 @@ __global__ void __launch_bounds__(1024) lanczos_zbatch(KParams P)
                             al = gam[ch] * sa;
                             share = fma(al, sa, share);
                         }
-                        for (int q = s; q <= t; q++) alpha[(q * B + b) * NLM + ch] = al;
+                        a6[ch] = al;
                     }
+                }
+                // the scatter's column-independent coefficients of the atom's images
+                // for node b: chi alpha = t0 K0 + t2 (xy Kxy + xx Kxx + yy Kyy)
+                // + t2 z (y Ky + x Kx) + t2 z^2 (-(Kxx + Kyy)), (x, y) the column
+                // relative to the image centre
+                {
+                    const T C = T(1.092548430592079), C20 = T(0.315391565252520), C22 = T(0.546274215296040);
+                    const T a1 = (T)a6[1], a2 = (T)a6[2], a3 = (T)a6[3], a4 = (T)a6[4], a5 = (T)a6[5];
+                    const T k6[NLM] = {(T)a6[0], C * a1, fma(C22, a5, -(C20 * a3)), fma(-C22, a5, -(C20 * a3)),
+                                       C * a2, C * a4};
+                    T *coef = reinterpret_cast<T *>(alpha);
+                    for (int q = s; q <= t; q++)
+#pragma unroll
+                        for (int k = 0; k < NLM; k++) coef[(q * B + b) * NLM + k] = k6[k];
                 }
                 pshare[li * B + b] = share;
             }
@@ -597,17 +611,16 @@ __global__ void __launch_bounds__(1024) lanczos_zbatch(KParams P)
                             const int64_t rb0 = toff[li] +
                                                 ((int64_t)((cube_lo[2] + z0 - lo.z) * dm.y + (gyc - lo.y)) * dm.x + (gxc - lo.x)) * 2;   // 2 rb0 in the basis table
                             const T xc = x0 - (T)cen.x, yc = y0 - (T)cen.y;
-                            const T C = T(1.092548430592079), C20 = T(0.315391565252520), C22 = T(0.546274215296040);
-                            const T cxy = C * (xc * yc), cy = C * yc, cx = C * xc, qq = -(xc * xc) - yc * yc,
-                                    cxxyy = C22 * (xc * xc - yc * yc);
+                            const T xy = xc * yc, xx = xc * xc, yy = yc * yc;
+                            const T *coef = reinterpret_cast<const T *>(alpha) + li * B * NLM;
                             T al0[B], p0[B], p1[B], p2[B];
 #pragma unroll
                             for (int b = 0; b < B; b++) {
-                                const double *al = alpha + (li * B + b) * NLM;
-                                al0[b] = (T)al[0];
-                                p0[b] = fma(cxy, (T)al[1], fma(cxxyy, (T)al[5], C20 * (qq * (T)al[3])));
-                                p1[b] = fma(cy, (T)al[2], cx * (T)al[4]);
-                                p2[b] = T(2) * C20 * (T)al[3];
+                                const T *k6 = coef + b * NLM;
+                                al0[b] = k6[0];
+                                p0[b] = fma(xy, k6[1], fma(xx, k6[2], yy * k6[3]));
+                                p1[b] = fma(yc, k6[4], xc * k6[5]);
+                                p2[b] = -(k6[2] + k6[3]);
                             }
                             const V4 *r4 = reinterpret_cast<const V4 *>(bas + 2 * rb0);
                             V4 t4[UC];
