@@ -131,6 +131,9 @@ __global__ void __launch_bounds__(1024) lanczos_zbatch(KParams P)
     // still runs
     __shared__ double s_sv[4], s_svp[4], s_bprev[4];
     __shared__ int s_nq[4], s_st[4], s_alive;
+    // per scatter chunk of UC union planes: the images whose z range meets it
+    constexpr int kMaxChunks = 16;
+    __shared__ unsigned s_zmask[kMaxChunks][MW];
 
     const int nx = P.nloc[0], ny = P.nloc[1], nz = P.nloc[2];
     const int ncol = nx * ny;
@@ -252,6 +255,25 @@ __global__ void __launch_bounds__(1024) lanczos_zbatch(KParams P)
                 if (w == (li >> 5) && in) cmask[w] |= 1u << (li & 31);
         }
         const int64_t eplane = (int64_t)P.extn[1] * P.extn[0];
+        // ---- per scatter chunk: bitmask of the images whose z range meets it, so a
+        // column's chunk visits only those (all images when the cube is too tall)
+        const int nch = (NZ + UC - 1) / UC;
+        const bool zmasked = nch <= kMaxChunks;
+        if (zmasked)
+            for (int e = tid; e < nch * MW; e += WG) {
+                const int c = e / MW, w = e - c * MW;
+                const int z0 = c * UC, z1 = z0 + UC - 1;
+                unsigned bits = 0u;
+                for (int k = 0; k < 32; k++) {
+                    const int li = w * 32 + k;
+                    if (li >= nimg) break;
+                    const int4 lo = tlo[li], dm = tdim[li];
+                    const int zlo = max(lo.z, cube_lo[2]) - cube_lo[2];
+                    const int zhi = min(lo.z + dm.z - 1, cube_lo[2] + NZ - 1) - cube_lo[2];
+                    if (zhi >= z0 && zlo <= z1) bits |= 1u << k;
+                }
+                s_zmask[c][w] = bits;
+            }
         // ---- Lanczos start: v_b = e_center of node b (union plane zcen + b), vprev = 0
         for (size_t p = tid; p < 2 * vlen; p += WG) buf0[p] = T(0);
         __syncthreads();
@@ -600,6 +622,7 @@ __global__ void __launch_bounds__(1024) lanczos_zbatch(KParams P)
                         unsigned m = cmask[0];
 #pragma unroll
                         for (int q = 1; q < MW; q++) m = (wd == q) ? cmask[q] : m;
+                        if (zmasked) m &= s_zmask[z0 / UC][wd];
                         while (m) {
                             const int li = (wd << 5) + __ffs(m) - 1;
                             m &= m - 1u;
