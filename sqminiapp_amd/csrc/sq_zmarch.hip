@@ -135,6 +135,9 @@ lanczos_zmarch(KParams P)
     __shared__ double s_g[kMatsubara][2];
     __shared__ int s_flag;
     __shared__ int s_near[kRecMax], s_nnear;
+    // per scatter chunk of UC planes: the images whose z range meets it
+    constexpr int kMaxChunks = 16;
+    __shared__ unsigned s_zmask[kMaxChunks][MW];
     for (;;) {
         if (tid == 0) {
             long r = -1;
@@ -231,6 +234,25 @@ lanczos_zmarch(KParams P)
             }
         }
         const int64_t eplane = (int64_t)P.extn[1] * P.extn[0];
+        // ---- per scatter chunk: bitmask of the images whose z range meets it, so a
+        // column's chunk visits only those (all images when the cube is too tall)
+        const int nch = (nz + UC - 1) / UC;
+        const bool zmasked = nch <= kMaxChunks;
+        if (zmasked)
+            for (int e = tid; e < nch * MW; e += WG) {
+                const int c = e / MW, w = e - c * MW;
+                const int z0 = c * UC, z1 = z0 + UC - 1;
+                unsigned bits = 0u;
+                for (int q = 0; q < 32; q++) {
+                    const int li = w * 32 + q;
+                    if (li >= nimg) break;
+                    const int4 lo = tlo[li], dm = tdim[li];
+                    const int zlo = max(lo.z, cube_lo[2]) - cube_lo[2];
+                    const int zhi = min(lo.z + dm.z - 1, cube_lo[2] + nz - 1) - cube_lo[2];
+                    if (zhi >= z0 && zlo <= z1) bits |= 1u << q;
+                }
+                s_zmask[c][w] = bits;
+            }
         // ---- Lanczos start: v = e_center, vprev = 0
         for (int p = tid; p < nc; p += WG) { buf0[p] = T(0); buf1[p] = T(0); }
         __syncthreads();
@@ -462,6 +484,7 @@ lanczos_zmarch(KParams P)
                         unsigned m = cmask[k][0];
 #pragma unroll
                         for (int q = 1; q < MW; q++) m = (wd == q) ? cmask[k][q] : m;
+                        if (zmasked) m &= s_zmask[z0 / UC][wd];
                         while (m) {
                             const int li = (wd << 5) + __ffs(m) - 1;
                             m &= m - 1u;
