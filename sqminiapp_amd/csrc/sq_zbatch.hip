@@ -75,30 +75,6 @@ __device__ __forceinline__ void block_sum_vec(double v[B], double *red)
     __syncthreads();
 }
 
-template <int B, int NW>
-__device__ __forceinline__ void block_csum_vec(CSum c[B], double v[B], double *red)
-{
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-#pragma unroll
-    for (int b = 0; b < B; b++) {
-        const CSum s = wave_csum(c[b]);
-        v[b] = (double)s.hi + (double)s.lo;
-    }
-    if (lane == 0)
-#pragma unroll
-        for (int b = 0; b < B; b++) red[b * NW + wave] = v[b];
-    __syncthreads();
-    if (threadIdx.x < B) {
-        double s = 0.0;
-        for (int w = 0; w < NW; w++) s += red[threadIdx.x * NW + w];
-        red[B * NW + threadIdx.x] = s;
-    }
-    __syncthreads();
-#pragma unroll
-    for (int b = 0; b < B; b++) v[b] = red[B * NW + b];
-    __syncthreads();
-}
-
 // One workgroup of 1024 threads per batch, one (x,y) column per thread.
 // UG: z-planes per gather trip,
 // UC: planes per scatter/update chunk.
@@ -501,10 +477,11 @@ __global__ void __launch_bounds__(1024) lanczos_zbatch(KParams P)
             for (int b = 0; b < B; b++) pn[b] = red[(B + 1) * NW + b];
             __syncthreads();
             // ---- local part, marching the union planes: u_b = (L + diag) v_b into Bu and <v_b, u_b>
+            // <v_b, u_b> and later ||w_b||^2: in FP32 each product is exact in FP64
+            // (24 x 24 bits), so the sums are FP64 sums of exact products
             double pa[B];
-            CSum ca[B];
 #pragma unroll
-            for (int b = 0; b < B; b++) { pa[b] = 0.0; ca[b] = CSum{0.f, 0.f}; }
+            for (int b = 0; b < B; b++) pa[b] = 0.0;
             T win[B][W];
             T svr[B];
 #pragma unroll
@@ -569,8 +546,7 @@ __global__ void __launch_bounds__(1024) lanczos_zbatch(KParams P)
                             for (int b = 0; b < B; b++) {
                                 const T vc = win[b][(s + O) % W];
                                 vset<VB, T>(uo, b, sm[b]);
-                                if constexpr (sizeof(T) == 8) pa[b] = fma(vc, sm[b], pa[b]);
-                                else ca[b] = csum_add_prod(ca[b], vc, sm[b]);
+                                pa[b] = fma((double)vc, (double)sm[b], pa[b]);
                             }
                             *reinterpret_cast<VB *>(Bu + (colB + zo)) = uo;
                         }
@@ -584,13 +560,9 @@ __global__ void __launch_bounds__(1024) lanczos_zbatch(KParams P)
                 }
             }
             double a[B];
-            if constexpr (sizeof(T) == 8) {
 #pragma unroll
-                for (int b = 0; b < B; b++) a[b] = pa[b];
-                block_sum_vec<B, NW>(a, red);
-            } else {
-                block_csum_vec<B, NW>(ca, a, red);
-            }
+            for (int b = 0; b < B; b++) a[b] = pa[b];
+            block_sum_vec<B, NW>(a, red);
 #pragma unroll
             for (int b = 0; b < B; b++) a[b] += pn[b];
             // ---- per column, chunks of UC union planes in registers: local part
@@ -601,9 +573,8 @@ __global__ void __launch_bounds__(1024) lanczos_zbatch(KParams P)
             for (int b = 0; b < B; b++) { ta[b] = T(-a[b]) * (T)s_sv[b]; tb[b] = T(-s_bprev[b]) * (T)s_svp[b]; }
             const int alive = s_alive;
             double pb[B];
-            CSum cb[B];
 #pragma unroll
-            for (int b = 0; b < B; b++) { pb[b] = 0.0; cb[b] = CSum{0.f, 0.f}; }
+            for (int b = 0; b < B; b++) pb[b] = 0.0;
             if (has) {
                 const T x0 = (T)(P.hgrid[0] * gxc - P.range[0]);
                 const T y0 = (T)(P.hgrid[1] * gyc - P.range[1]);
@@ -682,7 +653,7 @@ __global__ void __launch_bounds__(1024) lanczos_zbatch(KParams P)
                                 w = fmaf(ta[b], vget<VB, T>(vr, b), uu[b][u]);
                                 w = fmaf(tb[b], vget<VB, T>(vpr, b), w);
                                 w = in ? w : T(0);
-                                cb[b] = csum_add_prod(cb[b], w, w);
+                                pb[b] = fma((double)w, (double)w, pb[b]);
                             }
                             vset<VB, T>(wo, b, w);
                         }
@@ -691,13 +662,9 @@ __global__ void __launch_bounds__(1024) lanczos_zbatch(KParams P)
                 }
             }
             double bb[B];
-            if constexpr (sizeof(T) == 8) {
 #pragma unroll
-                for (int b = 0; b < B; b++) bb[b] = pb[b];
-                block_sum_vec<B, NW>(bb, red);
-            } else {
-                block_csum_vec<B, NW>(cb, bb, red);
-            }
+            for (int b = 0; b < B; b++) bb[b] = pb[b];
+            block_sum_vec<B, NW>(bb, red);
             double bv[B];
 #pragma unroll
             for (int b = 0; b < B; b++) bv[b] = sqrt(bb[b]);
