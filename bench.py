@@ -314,7 +314,7 @@ def main():
         counters32 = None
     mfma = load_json("lanczos_zbatch_c128_fp32_mfma_counters.json") if args.config == "C128" else None
     out["fp32_form"] = {
-        "form": "FP32 Lanczos, compensated (Dot2) dots, FP64 QL / Fermi level / sums, the autotuner's FP32 kernel",
+        "form": "FP32 Lanczos, compensated dots (FP64 sums of exact FP32 products), FP64 QL / Fermi level / sums, the autotuner's FP32 kernel",
         "kernel": {"batch": I.batch[1], "gather_trip": I.gather_trip[1],
                    "gather": "matrix cores" if I.batch[1] > 1 and I.gather_trip[1] >= 4 else "vector ALU"},
         "value": round(nodes32 / dt32, 1), "unit": "grid-nodes/s", "speedup_vs_fp64": round(nodes32 / dt32 / value, 3),
