@@ -391,25 +391,27 @@ __global__ void __launch_bounds__(1024) lanczos_zbatch(KParams P)
                             }
                         }
                     }
-                    // butterfly over all lanes, node by node; lane 0 writes
+                    // butterfly over all lanes, node by node, in the working precision (one
+                    // lane exchange a level in FP32); lane 0 writes
 #pragma unroll
                     for (int b = 0; b < B; b++)
 #pragma unroll
                         for (int ch = 0; ch < NLM; ch++) {
-                            double sv = (double)accb[b][ch];
+                            T sv = accb[b][ch];
 #pragma unroll
                             for (int off = 1; off < 64; off <<= 1) sv += __shfl_xor(sv, off, 64);
-                            if (lane == 0 && ch < nlm) A[(li * B + b) * NLM + ch] = sv * (double)(T)s_sv[b];
+                            if (lane == 0 && ch < nlm) A[(li * B + b) * NLM + ch] = (double)sv * (double)(T)s_sv[b];
                         }
                     continue;
                 }
-                // butterfly over the lanes of one node (lane bits 2..5); lanes 0..B-1 end with node lane
+                // butterfly over the lanes of one node (lane bits 2..5) in the working
+                // precision; lanes 0..B-1 end with node lane
 #pragma unroll
                 for (int ch = 0; ch < NLM; ch++) {
-                    double s = (double)acc[ch];
+                    T s = acc[ch];
 #pragma unroll
                     for (int off = 4; off < 64; off <<= 1) s += __shfl_xor(s, off, 64);
-                    if (lane < B && ch < nlm) A[(li * B + lane) * NLM + ch] = s * (double)(T)s_sv[lane];   // the sums used the stored v
+                    if (lane < B && ch < nlm) A[(li * B + lane) * NLM + ch] = (double)s * (double)(T)s_sv[lane];   // the sums used the stored v
                 }
             }
             __syncthreads();
