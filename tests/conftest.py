@@ -1,3 +1,4 @@
+import contextlib
 import os
 import sys
 
@@ -98,3 +99,32 @@ def rel_err(a, b):
     a = np.asarray(a, float)
     b = np.asarray(b, float)
     return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+KERNEL_ENV = ("SQ_BATCH", "SQ_ZMARCH_WG", "SQ_BATCH_TRIP")
+
+
+@contextlib.contextmanager
+def kernel_like(info, prec):
+    """Contexts created inside the block run, for precision `prec`, the kernel
+    that `info` reports.  Every context creation times the kernel candidates
+    again, and where two run within noise of each other a second context may
+    pick the other one: equally valid, but its sums round differently, so a
+    bit-for-bit comparison across contexts pins the kernel first."""
+    b = info.batch[prec]
+    env = {"SQ_BATCH": str(b)}
+    if b == 1:
+        env["SQ_ZMARCH_WG"] = str(info.workgroup[prec])
+    else:
+        env["SQ_BATCH_TRIP"] = str(info.gather_trip[prec])
+    old = {k: os.environ.get(k) for k in KERNEL_ENV}
+    for k in KERNEL_ENV:
+        os.environ.pop(k, None)
+    os.environ.update(env)
+    try:
+        yield
+    finally:
+        for k in KERNEL_ENV:
+            os.environ.pop(k, None)
+            if old[k] is not None:
+                os.environ[k] = old[k]

@@ -78,9 +78,11 @@ def test_mg1_batched_collapsed_mode(monkeypatch, golden):
     assert rel_err(a[s], golden["ref_a"]) < 1e-10 and rel_err(b[s], golden["ref_b"]) < 1e-10
 
 
-@pytest.mark.parametrize("prec,B", [(sq.SQ_FP64, 2), (sq.SQ_FP32, 4)])
-def test_batched_results_do_not_depend_on_the_batch(monkeypatch, prec, B):
-    with _ctx(monkeypatch, B, phantom_atom=True) as c:
+@pytest.mark.parametrize("prec,B,trip", [(sq.SQ_FP64, 2, 2), (sq.SQ_FP32, 4, 4), (sq.SQ_FP32, 4, 2)])
+def test_batched_results_do_not_depend_on_the_batch(monkeypatch, prec, B, trip):
+    """Per kernel build (batch size and gather engine fixed in both contexts)."""
+    with _ctx(monkeypatch, B, trip, phantom_atom=True) as c:
+        assert c.info.batch[prec] == B and c.info.gather_trip[prec] == trip
         c.run(sq.SQ_NL_KB, prec)
         lf_, wf, _, _ = c.results()
         # a range starting mid-plane and mid-batch: its nodes run in other batches
@@ -89,7 +91,8 @@ def test_batched_results_do_not_depend_on_the_batch(monkeypatch, prec, B):
         r = slice(5 * 198 + 17, 5 * 198 + 17 + 891)
         assert np.array_equal(l1, lf_[r]) and np.array_equal(w1, wf[r])
     # a rank box of a 4-rank split
-    with _ctx(monkeypatch, B, phantom_atom=True, rank_box=(0, 9, 9, 11, 9, 8)) as rc:
+    with _ctx(monkeypatch, B, trip, phantom_atom=True, rank_box=(0, 9, 9, 11, 9, 8)) as rc:
+        assert rc.info.batch[prec] == B and rc.info.gather_trip[prec] == trip
         rc.run(sq.SQ_NL_KB, prec)
         lr, wr, _, _ = rc.results()
     ids = [(k * 18 + j) * 11 + i for k in range(9, 17) for j in range(9, 18) for i in range(11)]
@@ -178,7 +181,7 @@ def test_batched_early_exit_equals_shorter_runs(monkeypatch, tmp_path):
     checked = 0
     for s in sorted(set(sizes.tolist()))[:3]:
         ids = np.nonzero(sizes == s)[0][:4]
-        with _ctx(monkeypatch, 2, deck=(d, n), nqp=int(s)) as cs:
+        with _ctx(monkeypatch, 2, c.info.gather_trip[0], deck=(d, n), nqp=int(s)) as cs:
             for i in ids:
                 cs.run(sq.SQ_NL_KB, sq.SQ_FP64, begin=int(i), count=1)
                 ls, ws, mn, mx = cs.results()

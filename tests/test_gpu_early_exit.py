@@ -10,7 +10,7 @@ against closed forms, tests/test_matsubara.py.)
 import numpy as np
 import pytest
 
-from conftest import MG1, write_variant
+from conftest import MG1, kernel_like, write_variant
 
 import sqminiapp_amd as sq
 
@@ -32,7 +32,9 @@ def test_stopped_nodes_equal_shorter_fixed_runs(tmp_path):
     checked = 0
     for s in sorted(set(sizes.tolist()))[:3]:
         ids = np.nonzero(sizes == s)[0][:4]
-        with sq.Context(d, n, nqp=int(s)) as cs:
+        with kernel_like(c.info, sq.SQ_FP64):
+            cs = sq.Context(d, n, nqp=int(s))
+        with cs:
             for i in ids:
                 cs.run(sq.SQ_NL_KB, sq.SQ_FP64, begin=int(i), count=1)
                 ls, ws, mn, mx = cs.results()

@@ -17,6 +17,8 @@ import os
 import numpy as np
 import pytest
 
+from conftest import kernel_like
+
 import sqminiapp_amd as sq
 from sqminiapp_amd import decks
 
@@ -98,6 +100,7 @@ def test_c128_early_exit_equals_fixed_shorter_runs(c128, monkeypatch, tol):
     monkeypatch.setenv("SQ_BATCH", "4")
     with sq.Context(d, name) as ctx:
         assert ctx.info.batch[1] == 4
+        info = ctx.info
         n = ctx.info.n_nodes
         begin, count = n // 2 - 8000, 16000
         ctx.run(sq.SQ_NL_KB, sq.SQ_FP32)
@@ -114,8 +117,10 @@ def test_c128_early_exit_equals_fixed_shorter_runs(c128, monkeypatch, tol):
         if s == 80:
             continue
         ids = np.nonzero(sizes == s)[0][:3]
-        with sq.Context(d, name, nqp=int(s)) as cs:
-            assert cs.info.batch[1] == 4
+        with kernel_like(info, sq.SQ_FP32):
+            cs = sq.Context(d, name, nqp=int(s))
+        with cs:
+            assert cs.info.batch[1] == 4 and cs.info.gather_trip[1] == info.gather_trip[1]
             for i in ids:
                 cs.run(sq.SQ_NL_KB, sq.SQ_FP32, begin=begin + int(i), count=1)
                 ls, ws, mn, mx = cs.results()

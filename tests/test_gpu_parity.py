@@ -11,7 +11,7 @@ import os
 import numpy as np
 import pytest
 
-from conftest import GOLDEN, MG1, rel_err, write_variant
+from conftest import GOLDEN, MG1, kernel_like, rel_err, write_variant
 
 import sqminiapp_amd as sq
 
@@ -162,8 +162,10 @@ def test_runs_are_deterministic_and_partition_independent(ctx):
     ctx.run(sq.SQ_NL_KB, sq.SQ_FP64, begin=1683, count=891)
     l2, w2, _, _ = ctx.results()
     assert np.array_equal(l1, l2) and np.array_equal(w1, w2)
-    # the same nodes owned by a rank box (0,9,9 | 11,9,8) of a 4-rank split
-    with sq.Context(MG1, "Mg1", phantom_atom=True, rank_box=(0, 9, 9, 11, 9, 8)) as rc:
+    # the same nodes owned by a rank box (0,9,9 | 11,9,8) of a 4-rank split, same kernel
+    with kernel_like(ctx.info, sq.SQ_FP64):
+        rc = sq.Context(MG1, "Mg1", phantom_atom=True, rank_box=(0, 9, 9, 11, 9, 8))
+    with rc:
         rc.run(sq.SQ_NL_KB, sq.SQ_FP64)
         lr, wr, _, _ = rc.results()
     ctx.run(sq.SQ_NL_KB, sq.SQ_FP64)

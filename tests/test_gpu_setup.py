@@ -8,7 +8,7 @@ import time
 import numpy as np
 import pytest
 
-from conftest import MG1
+from conftest import MG1, kernel_like
 
 import sqminiapp_amd as sq
 from sqminiapp_amd import decks
@@ -22,7 +22,8 @@ def _contexts(d, name, **kw):
     t1 = time.perf_counter()
     os.environ["SQ_HOST_TABLES"] = "1"
     try:
-        host = sq.Context(d, name, **kw)
+        with kernel_like(dev.info, sq.SQ_FP64):   # the same kernel, so results compare bit for bit
+            host = sq.Context(d, name, **kw)
     finally:
         del os.environ["SQ_HOST_TABLES"]
     t2 = time.perf_counter()
