@@ -73,8 +73,7 @@ struct sq_ctx {
     long batch_cap = 0, n_batches = 0, bat_begin = -1, bat_count = -1;
     int bat_B = 0;
     int *probe_ids = nullptr;     // the autotuner's node set (one-node kernel form)
-    double *bas64 = nullptr;      // column basis tables of the batched kernel
-    float *bas32 = nullptr;
+    float *bas32 = nullptr;       // column basis table of the batched kernel's FP32 form
     hipStream_t stream = nullptr;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     // device tables
@@ -152,7 +151,7 @@ void free_ctx(sq_ctx *c)
     void *ptrs[] = {c->img_dimp, c->rad64, c->rad32, c->img_rad, c->img_cen, c->chi64, c->chi32, c->img_lo, c->img_dim, c->img_off, c->img_atom, c->img_type, c->gamma,
                     c->cov_off, c->cov_img, c->diag64, c->diag32, c->c64, c->c32, c->scratch, c->a, c->b,
                     c->lam, c->w, c->lmin, c->lmax, c->qlws, c->status, c->partial, c->partial2, c->out,
-                    c->queue, c->nsteps, c->batches, c->probe_ids, c->bas64, c->bas32};
+                    c->queue, c->nsteps, c->batches, c->probe_ids, c->bas32};
     void *scf_ptrs[] = {c->scf.rho, c->scf.rho_out, c->scf.b, c->scf.phi, c->scf.eps, c->scf.vxc, c->scf.veff, c->scf.r,
                         c->scf.p, c->scf.Ap, c->scf.f, c->scf.x_prev, c->scf.f_prev, c->scf.dX, c->scf.dF, c->scf.g,
                         c->scf.s, c->scf.part, c->scf.ku, c->scf.ky};
@@ -500,12 +499,12 @@ int create_device(sq_ctx *c, int device)
         c->zb_lds[B] = cap <= 256 && z <= 150 * 1024 ? z : 0;
     }
     if (c->zbatch) {
-        // (t0, t2, t2 z, t2 z^2) per image box point, from the radial factors
+        // FP32: (t0, t2, t2 z, t2 z^2) per image box point, from the radial
+        // factors (the FP64 form forms them in the kernel)
         const int64_t n4 = std::max<int64_t>(1, 2 * S.rad_total);
-        HIP_TRY(hipMalloc((void **)&c->bas64, n4 * sizeof(double)));
         HIP_TRY(hipMalloc((void **)&c->bas32, n4 * sizeof(float)));
         HIP_TRY(launch_basis_tables(c->rad64, M, c->img_lo, c->img_dim, c->img_rad, c->img_cen, S.h[2], S.range[2],
-                                    c->bas64, c->bas32, c->stream));
+                                    c->bas32, c->stream));
         HIP_TRY(hipStreamSynchronize(c->stream));
     }
     return SQ_OK;
@@ -631,7 +630,7 @@ KParams make_batch_params(const sq_ctx *c, int mode, int precision, int B, const
     P.lds_plane = off[8];
     P.lds_run = off[9];
     P.lds_share = off[10];
-    P.basis = precision == SQ_FP64 ? (const void *)c->bas64 : (const void *)c->bas32;
+    P.basis = precision == SQ_FP64 ? nullptr : (const void *)c->bas32;
     P.img_cap = c->zb_cap[B];
     P.batch = batch;
     P.node_count = n_batch;

@@ -67,7 +67,7 @@ struct KParams {
     // batched kernel: node_count batches of up to B nodes (i, j, k .. k+nb-1),
     // each {id of its lowest node, nb}
     const int2 *batch;
-    const void *basis;           // (t0, t2, t2 z, t2 z^2) per image box point (element type = precision)
+    const void *basis;           // FP32: (t0, t2, t2 z, t2 z^2) per image box point (FP64 reads rad)
     int lds_run, lds_share;      // batched kernel: per-image atom run and type; per-atom shares of <v, V_nl v>
     int plane_px;                // z-plane row pitch in LDS: the cube's nx plus halo and bank padding
 };
@@ -80,7 +80,7 @@ bool zbatch_supported(int precision, int B, int ug);
 hipError_t launch_zbatch(const KParams &P, int precision, int B, int ug, int grid, size_t lds, hipStream_t st);
 int zbatch_occupancy(const KParams &P, int precision, int B, int ug, size_t lds);
 hipError_t launch_basis_tables(const double *rad, int n_images, const int4 *lo, const int4 *dim, const int64_t *roff,
-                               const double4 *cen, double hz, double rz, double *b64, float *b32, hipStream_t st);
+                               const double4 *cen, double hz, double rz, float *b32, hipStream_t st);
 hipError_t launch_apply(const KParams &P, long id, const double *x, double *y, int nonlocal_only, size_t lds,
                         hipStream_t st);
 hipError_t launch_ql(const double *a, const double *b, int n, int stride, long count, double *lam, double *w,

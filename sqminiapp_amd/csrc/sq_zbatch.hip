@@ -26,7 +26,11 @@
 // instruction (v_mfma_f64_4x4x4_4b) runs at the FP64 vector rate; it was
 // measured 10-35 % slower than the vector form and is not built.  The six
 // channel sums of a node follow from S and the column constants, then a
-// butterfly over the columns.
+// butterfly over the columns.  FP32 reads the four basis values of a point
+// from a precomputed table (16 bytes); FP64 reads the point's radial pair
+// (t0, t2) (16 bytes, half the FP64 table) and forms t2 z and t2 z^2 in
+// registers with the table builder's own operations, so the values, and every
+// result, are the same bit for bit.
 //
 // Per H application: gather -> alpha per node -> local march (stencil
 // from a 13-deep register ring per node and an LDS plane of the batch) ->
@@ -84,6 +88,7 @@ __global__ void __launch_bounds__(1024) lanczos_zbatch(KParams P)
     constexpr int WG = 1024, NW = WG / 64, O = 6, W = 2 * O + 1, NLM = 6;
     using VB = typename VecOf<T, B>::type;
     typedef T V4 __attribute__((ext_vector_type(4)));
+    typedef T V2 __attribute__((ext_vector_type(2)));
     extern __shared__ __align__(16) unsigned char smem[];
     int *img_list = reinterpret_cast<int *>(smem);
     int4 *tlo = reinterpret_cast<int4 *>(smem + P.lds_tlo);
@@ -126,7 +131,10 @@ __global__ void __launch_bounds__(1024) lanczos_zbatch(KParams P)
     T *__restrict__ buf1 = buf0 + vlen;
     T *__restrict__ Bu = buf1 + vlen;
     const T *__restrict__ diag_ext = reinterpret_cast<const T *>(P.diag_ext);
-    const T *__restrict__ bas = reinterpret_cast<const T *>(P.basis);   // (t0, t2, t2 z, t2 z^2) per image point
+    // FP32: (t0, t2, t2 z, t2 z^2) per image point; FP64: the radial pairs (t0, t2)
+    const T *__restrict__ bas = reinterpret_cast<const T *>(P.basis);
+    const T *__restrict__ rad = reinterpret_cast<const T *>(P.rad);
+    const double hz = P.hgrid[2], rz = P.range[2];
     if (tid < 21) {
         if constexpr (sizeof(T) == 8) s_w[tid / 7][tid % 7] = P.cw64[tid / 7][tid % 7];
         else s_w[tid / 7][tid % 7] = P.cw32[tid / 7][tid % 7];
@@ -342,7 +350,6 @@ __global__ void __launch_bounds__(1024) lanczos_zbatch(KParams P)
                     for (int b = 0; b < B; b++)
 #pragma unroll
                         for (int ch = 0; ch < NLM; ch++) accb[b][ch] = T(0);
-                    const V4 *bas4 = reinterpret_cast<const V4 *>(bas + 2 * toff[li]);
                     for (int q0 = 0; q0 < nxy; q0 += 64) {
                         const int q = q0 + lane;
                         const bool ok = q < nxy;
@@ -351,7 +358,7 @@ __global__ void __launch_bounds__(1024) lanczos_zbatch(KParams P)
                         const int gx = a0[0] + xi, gy = a0[1] + yi;
                         const int c = (gy - cube_lo[1]) * nx + (gx - cube_lo[0]);
                         const VB *vp = reinterpret_cast<const VB *>(Bv) + (size_t)(a0[2] - cube_lo[2]) * ncol + c;
-                        const V4 *rp = bas4 + ((int64_t)(a0[2] - lo.z) * dm.y + (gy - lo.y)) * dm.x + (gx - lo.x);
+                        const int64_t pix = ((int64_t)(a0[2] - lo.z) * dm.y + (gy - lo.y)) * dm.x + (gx - lo.x);
                         T sm[B][4];
 #pragma unroll
                         for (int b = 0; b < B; b++)
@@ -364,7 +371,16 @@ __global__ void __launch_bounds__(1024) lanczos_zbatch(KParams P)
 #pragma unroll
                             for (int u = 0; u < UZ; u++) {
                                 const bool in = ok && zi + u < n[2];
-                                tb[u] = in ? rp[(int64_t)(zi + u) * zs2] : V4{};
+                                const int64_t p = pix + (int64_t)(zi + u) * zs2;
+                                if constexpr (sizeof(T) == 8) {
+                                    // z of the plane: uniform across the wave
+                                    const V2 r = in ? reinterpret_cast<const V2 *>(rad + toff[li])[p] : V2{};
+                                    const double z = hz * (a0[2] + zi + u) - rz - cen.z;
+                                    const double t2z = r.y * z;
+                                    tb[u] = V4{r.x, r.y, t2z, t2z * z};
+                                } else {
+                                    tb[u] = in ? reinterpret_cast<const V4 *>(bas + 2 * toff[li])[p] : V4{};
+                                }
                                 vv[u] = in ? vp[(size_t)(zi + u) * ncol] : VB{};
                             }
 #pragma unroll
@@ -604,8 +620,9 @@ __global__ void __launch_bounds__(1024) lanczos_zbatch(KParams P)
                             const int zhi = min(lo.z + dm.z - 1, cube_lo[2] + NZ - 1) - cube_lo[2];
                             if (zhi < z0 || zlo > z0 + UC - 1) continue;
                             const double4 cen = tcen[li];
+                            // the point's radial pair; its basis quadruple sits at 2 rb0 in the table
                             const int64_t rb0 = toff[li] +
-                                                ((int64_t)((cube_lo[2] + z0 - lo.z) * dm.y + (gyc - lo.y)) * dm.x + (gxc - lo.x)) * 2;   // 2 rb0 in the basis table
+                                                ((int64_t)((cube_lo[2] + z0 - lo.z) * dm.y + (gyc - lo.y)) * dm.x + (gxc - lo.x)) * 2;
                             const T xc = x0 - (T)cen.x, yc = y0 - (T)cen.y;
                             const T xy = xc * yc, xx = xc * xc, yy = yc * yc;
                             const T *coef = reinterpret_cast<const T *>(alpha) + li * B * NLM;
@@ -618,12 +635,18 @@ __global__ void __launch_bounds__(1024) lanczos_zbatch(KParams P)
                                 p1[b] = fma(yc, k6[4], xc * k6[5]);
                                 p2[b] = -(k6[2] + k6[3]);
                             }
-                            const V4 *r4 = reinterpret_cast<const V4 *>(bas + 2 * rb0);
                             V4 t4[UC];
 #pragma unroll
                             for (int u = 0; u < UC; u++) {
                                 const bool ok = z0 + u >= zlo && z0 + u <= zhi;
-                                t4[u] = ok ? r4[u * (dm.x * dm.y)] : V4{};
+                                if constexpr (sizeof(T) == 8) {
+                                    const V2 r = ok ? reinterpret_cast<const V2 *>(rad + rb0)[u * (dm.x * dm.y)] : V2{};
+                                    const double z = hz * (cube_lo[2] + z0 + u) - rz - cen.z;
+                                    const double t2z = r.y * z;
+                                    t4[u] = V4{r.x, r.y, t2z, t2z * z};
+                                } else {
+                                    t4[u] = ok ? reinterpret_cast<const V4 *>(bas + 2 * rb0)[u * (dm.x * dm.y)] : V4{};
+                                }
                             }
 #pragma unroll
                             for (int u = 0; u < UC; u++) {
@@ -743,13 +766,14 @@ __global__ void __launch_bounds__(1024) lanczos_zbatch(KParams P)
     }
 }
 
-// Column basis tables of the batched kernel: for every point of an s+d image
-// box, (t0, t2, t2 z, t2 z^2) with t0, t2 the radial factors and z the
-// point's height above the image centre (lsdft_nl.cc:174-190), in FP64 and
-// FP32.  Entry 4 p of image g starts at 2 img_rad[g].
+// Column basis table of the batched kernel's FP32 form: for every point of an
+// s+d image box, (t0, t2, t2 z, t2 z^2) with t0, t2 the radial factors and z
+// the point's height above the image centre (lsdft_nl.cc:174-190), formed in
+// FP64 and rounded once.  Entry 4 p of image g starts at 2 img_rad[g].  (The
+// FP64 form forms the same values in the kernel from the radial pairs.)
 __global__ void basis_tables(const double *__restrict__ rad, const int4 *__restrict__ lo, const int4 *__restrict__ dim,
                              const int64_t *__restrict__ roff, const double4 *__restrict__ cen, double hz, double rz,
-                             double *__restrict__ b64, float *__restrict__ b32)
+                             float *__restrict__ b32)
 {
     const int g = blockIdx.x;
     const int4 l = lo[g], d = dim[g];
@@ -760,18 +784,15 @@ __global__ void basis_tables(const double *__restrict__ rad, const int4 *__restr
         const double z = hz * (l.z + zi) - rz - cen[g].z;
         const double t0 = rad[o + 2 * p], t2 = rad[o + 2 * p + 1];
         const double v[4] = {t0, t2, t2 * z, (t2 * z) * z};
-        for (int k = 0; k < 4; k++) {
-            b64[2 * o + 4 * p + k] = v[k];
-            b32[2 * o + 4 * p + k] = (float)v[k];
-        }
+        for (int k = 0; k < 4; k++) b32[2 * o + 4 * p + k] = (float)v[k];
     }
 }
 
 hipError_t launch_basis_tables(const double *rad, int n_images, const int4 *lo, const int4 *dim, const int64_t *roff,
-                               const double4 *cen, double hz, double rz, double *b64, float *b32, hipStream_t st)
+                               const double4 *cen, double hz, double rz, float *b32, hipStream_t st)
 {
     if (n_images <= 0) return hipSuccess;
-    hipLaunchKernelGGL(basis_tables, dim3(n_images), dim3(256), 0, st, rad, lo, dim, roff, cen, hz, rz, b64, b32);
+    hipLaunchKernelGGL(basis_tables, dim3(n_images), dim3(256), 0, st, rad, lo, dim, roff, cen, hz, rz, b32);
     return hipGetLastError();
 }
 
